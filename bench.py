@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Light-map frame benchmark on one MI355X.
+
+Workload: two displaced icospheres (subdivision 4 each, 10,240 glass triangles, radial noise
+sigma 0.02) over a diffuse plane; 1024 x 1024 rays from a pinhole at the point light, one seeded
+wavelength per ray, Cauchy glass (A 1.5, B 0.004 um^2), one-level bounding-volume hitter, 4
+bounces with dual-number differentials, splat into a 1024 x 1024 RGB float32 light map.
+A step is one whole frame: clear map -> pinhole rays -> bounce loop (hit, evolve, stable
+compaction) -> realigning hit -> final evolution + splat, all device-resident, one frame per
+launch sequence ("plain" path).
+
+Prints one JSON line (the last line of stdout).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s + p50/p99 frame latency, served-batched vs plain HIP, 1x MI355X"
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_TRI_TEST = 40  # distance_to_plane + in_triangle + p = o + t d, per-triangle terms hoisted
+FLOP_PER_BOX_TEST = 12  # slab test with precomputed reciprocals
+BV_DEPTH = 6
+
+
+class Hip:
+    """The few raw HIP event calls needed to time kernels inside the C ABI's launch sequence."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def events(self, n):
+        out = (C.c_void_p * n)()
+        for i in range(n):
+            e = C.c_void_p()
+            if self.h.hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            out[i] = e
+        return out
+
+    def elapsed_ms(self, a, b) -> float:
+        ms = C.c_float()
+        if self.h.hipEventElapsedTime(C.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+    def destroy(self, evs):
+        for e in evs:
+            self.h.hipEventDestroy(e)
+
+
+def build_workload(device="cuda"):
+    from splat_amd import abi, api, bv, scene
+
+    sc = scene.scene_c2(seed=2)
+    cam = scene.camera_c2(seed=2, jitter=0.025)
+    bvs, mem = bv.bv_partition(sc.ftris, BV_DEPTH)
+    tris_d = api.to_device(sc.ftris, device)
+    hitter = api.HIPBVHitter(tris_d, bvs, mem)
+    params = abi.run_params(4, sc.first_diffuse, math.nan, seed=2024)
+    lm = abi.lightmap_params(1024, 1024)
+    frame = api.Frame(cam, hitter, tris_d, params, lm, device)
+    return sc, cam, (bvs, mem), hitter, params, lm, frame
+
+
+def cpu_baseline(sc, cam, bvmem, params, lm, budget_s: float = 12.0):
+    """The oracle's FP32 C++ restatement of the same frame (BV-culled hitter, OpenMP), timed on
+    this host on an image-wide strided sample of the frame's rays."""
+    from oracle import oracle as O
+    from splat_amd import bv
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    counts, mat, stride_m = bv.pack_bv_tris(bvmem[1], sc.n_tris)
+    R_full = cam.height * cam.width * cam.samples_per_pixel
+    t = time.perf_counter()
+    _, n = O.trace_sample(cam, 512, 0, sc.ftris, params, lm, (counts, mat, stride_m), threads)
+    per_ray = (time.perf_counter() - t) / max(n, 1)
+    stride = max(1, int(math.ceil(R_full * per_ray / budget_s)))
+    t = time.perf_counter()
+    _, n = O.trace_sample(cam, stride, 0, sc.ftris, params, lm, (counts, mat, stride_m), threads)
+    dt = time.perf_counter() - t
+    return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th ray of the 1024x1024 frame ({n} rays, {dt:.1f} s): oracle FP32 C++ "
+                      f"frame (same BV culling, 4 bounces, splat), OpenMP {threads} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from splat_amd import abi
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    sc, cam, bvmem, hitter, params, lm, frame = build_workload()
+    R = frame.R
+    for _ in range(args.warmup):
+        frame.trace()
+    torch.cuda.synchronize()
+
+    K = args.steps
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record()
+    for i in range(K):
+        frame.trace()
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    frame_ms = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])
+    value = R * K / wall / 1e6
+    if args.profile:
+        print(json.dumps({"frames": K, "ms_per_step": wall / K * 1e3}))
+        return 0
+
+    # ---- instrumented frames: hitter launches timed with events on the frame's stream ----
+    hip = Hip()
+    n_launch = params.depth + 1
+    M = 20
+    hev = hip.events(2 * n_launch * M)
+    per = 2 * n_launch
+    for m in range(M):
+        params.hit_events = C.addressof(hev) + m * per * C.sizeof(C.c_void_p)
+        params.n_hit_events = per
+        frame.trace()
+    torch.cuda.synchronize()
+    launch_ms = [hip.elapsed_ms(hev[m * per + 2 * j], hev[m * per + 2 * j + 1]) for m in range(M) for j in range(n_launch)]
+    params.hit_events, params.n_hit_events = None, 0
+    hip.destroy(hev)
+    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    hitter_frame_ms = float(np.sum(launch_ms)) / M
+
+    # ---- algorithmic work of the hitter (one counting frame, untimed) ----
+    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    hitter.hs.d_stats = stats.data_ptr()
+    frame.trace()
+    torch.cuda.synchronize()
+    hitter.hs.d_stats = None
+    box_tests, entered, alg_tests, exec_tests = (int(x) for x in stats.cpu().tolist())
+    flop_per_launch = (FLOP_PER_BOX_TEST * box_tests + FLOP_PER_TRI_TEST * alg_tests) / n_launch
+    achieved = flop_per_launch / avg_launch_s / 1e12
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, bvmem, params, lm)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded procedural scene and pinhole rays (no dataset)",
+        "config": {
+            "workload": "1 MP light-map frame: 2 displaced icospheres (subdiv 4, 10,240 glass tris) + diffuse plane, "
+                        "BV hitter (one-level, 2^6 leaves), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
+                        "light map; plain path (one frame per launch sequence)",
+            "rays_per_frame": R,
+            "triangles": sc.n_tris,
+            "bounces": params.depth,
+            "path": "plain",
+        },
+        "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
+        "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        "roofline": {
+            "kernel": "k_hit_bv (one-level BV hitter)",
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flop_per_launch": flop_per_launch,
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+        },
+        "hitter": {
+            "ms_per_frame": round(hitter_frame_ms, 4),
+            "box_tests_per_frame": box_tests,
+            "tri_tests_alg_per_frame": alg_tests,
+            "tri_tests_exec_per_frame": exec_tests,
+            "simd_efficiency": round(alg_tests / max(exec_tests, 1), 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

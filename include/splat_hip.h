@@ -1,0 +1,291 @@
+/*
+ * splat_hip.h -- C ABI of the MI355X (gfx950) per-frame tracing hot path of SPLAT.
+ *
+ * Every entry point is a plain C function over device pointers and sizes. It enqueues its work
+ * on the caller's HIP stream (passed as an opaque `void*`, NULL = the null stream) and returns
+ * an int32 status: 0 on success, a negative SPLAT_ERR_* code otherwise (splat_strerror() gives
+ * the message). Nothing throws across the ABI. Apart from the serving loop, no entry point
+ * allocates device memory: workspaces are sized by the *_bytes() helpers and owned by the
+ * caller. All layouts below are byte-for-byte the Julia isbits layouts of the reference, so a
+ * Julia caller passes `pointer(::CuArray)`-style device pointers with zero copies.
+ *
+ * Index conventions follow the reference: triangle indices are 1-based Julia indices, index 1 is
+ * the degenerate all-zero triangle that doubles as the "miss" sentinel (src/geo.jl:312,
+ * src/scenes/sphere.jl:36), triangles >= first_diffuse are diffuse (src/tracer.jl:186).
+ */
+#ifndef SPLAT_HIP_H
+#define SPLAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPLAT_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------------------------ */
+#define SPLAT_OK 0
+#define SPLAT_ERR_INVALID_ARG (-1) /* null pointer, negative size, bad enum               */
+#define SPLAT_ERR_WORKSPACE (-2)   /* caller workspace too small                          */
+#define SPLAT_ERR_HIP (-3)         /* a HIP runtime call failed                           */
+#define SPLAT_ERR_UNSUPPORTED (-4) /* parameter outside the supported range               */
+#define SPLAT_ERR_CAPACITY (-5)    /* a fixed-capacity table would overflow               */
+#define SPLAT_ERR_BUDGET (-6)      /* serving: request does not fit the HBM budget         */
+
+/* ---- ray status (src/geo.jl:78-80) ------------------------------------------------------- */
+#define SPLAT_RAY_DIFFUSE 0
+#define SPLAT_RAY_ACTIVE 1
+#define SPLAT_RAY_INFINITY 2
+
+/* ---- data layouts (Julia isbits, src/geo.jl:12-113, src/tracer.jl:313) ------------------- */
+
+/* ADRay, src/geo.jl:60-76. 136 bytes: 8 x vec3 (96 B), Bool @96, Int64 @104/@112,
+ * Float32 lambda/x/y @120/124/128, UInt8 status @132. */
+typedef struct splat_adray {
+  float pos[3];
+  float dpdl[3]; /* d pos / d lambda */
+  float dpdx[3]; /* d pos / d x      */
+  float dpdy[3]; /* d pos / d y      */
+  float dir[3];
+  float dddl[3]; /* d dir / d lambda */
+  float dddx[3];
+  float dddy[3];
+  uint8_t in_medium;
+  uint8_t _pad0[7];
+  int64_t ignore_tri; /* 1-based triangle index the ray just left (1 = none) */
+  int64_t dest;       /* sample id (camera_ray: pixel index, src/ray_generators.jl:32) */
+  float lambda;       /* wavelength, nm */
+  float x, y;         /* expansion point of the pixel-offset differentials */
+  uint8_t status;     /* SPLAT_RAY_* */
+  uint8_t _pad1[3];
+} splat_adray_t;
+
+/* Tri = SVector{4,R3}: [face normal, a, b, c] (src/geo.jl:16). 48 bytes. */
+typedef struct splat_tri {
+  float v[4][3];
+} splat_tri_t;
+
+/* n_tris element = Tuple{Int32,Tri} (src/tracer.jl:313). 52 bytes. */
+typedef struct splat_ntri {
+  int32_t idx;
+  float v[4][3];
+} splat_ntri_t;
+
+/* FTri = SVector{10,R3}: [n, a, b, c, n_a, n_b, n_c, uv_a, uv_b, uv_c] (src/geo.jl:18,
+ * src/geo.jl:337). 120 bytes. */
+typedef struct splat_ftri {
+  float v[10][3];
+} splat_ftri_t;
+
+/* BVBox (src/geo.jl:23-26), 24 bytes; Sphere (src/geo.jl:40-43), 16 bytes. */
+typedef struct splat_bvbox {
+  float min[3];
+  float max[3];
+} splat_bvbox_t;
+
+typedef struct splat_sphere {
+  float origin[3];
+  float radius;
+} splat_sphere_t;
+
+/* ---- optics ------------------------------------------------------------------------------ */
+#define SPLAT_IOR_REFERENCE_LINEAR 0 /* glass(lambda): 1.3 @400nm -> 1.4 @700nm (src/material.jl:1-9) */
+#define SPLAT_IOR_CAUCHY 1           /* n = A + B / lambda_um^2                                      */
+
+/* Reference-compatibility switches (both on reproduces src/tracer.jl:118-179 exactly). */
+#define SPLAT_COMPAT_DY_AT_X (1 << 0)       /* d dir / d y taken at y = r.x (src/tracer.jl:138-141, :166-169) */
+#define SPLAT_COMPAT_REFRACT_NORMAL_LAMBDA (1 << 1) /* refract's lambda tangent holds the normal fixed (:130-133) */
+#define SPLAT_COMPAT_REFERENCE (SPLAT_COMPAT_DY_AT_X | SPLAT_COMPAT_REFRACT_NORMAL_LAMBDA)
+
+typedef struct splat_optics {
+  int32_t ior_model; /* SPLAT_IOR_* */
+  float cauchy_a;    /* Cauchy A (dimensionless)          */
+  float cauchy_b;    /* Cauchy B (um^2)                   */
+  int32_t compat;    /* SPLAT_COMPAT_* bits               */
+} splat_optics_t;
+
+/* ---- prepared hit sets (the hitter's per-scene state, cf. src/alg_types.jl:10-90) ------- */
+#define SPLAT_HITTER_BF 0 /* brute force: StableHitter, EH1..EH6 (src/hitters.jl:24-310)         */
+#define SPLAT_HITTER_BV 1 /* one-level bounding volumes: DPBVHitter (src/hitters.jl:486-648)      */
+
+/* Host-side descriptor filled by splat_hitset_prepare_*(); the device arrays it points to live
+ * inside the caller's workspace. Plain data: copy it freely, it owns nothing. */
+typedef struct splat_hitset {
+  int32_t kind;          /* SPLAT_HITTER_*                                              */
+  int32_t n_tris;        /* N, including the degenerate index 1                          */
+  int32_t n_bv;          /* number of bounding volumes (BV only)                         */
+  int32_t max_members;   /* largest BV membership (BV only)                              */
+  int64_t total_members; /* sum of BV memberships (BV only)                              */
+  void* d_recs;          /* device: prepared triangle records, 64 B each                 */
+  void* d_boxes;         /* device: conservative member boxes, 32 B each (BV only)       */
+  void* d_ranges;        /* device: int32 (first record, count) per BV (BV only)          */
+  int32_t tune_waves;    /* triangle-split waves per workgroup (0 = autotuned default)   */
+  int32_t _pad;
+  /* Optional device uint64[4] counters, accumulated by every hitter launch when non-NULL:
+   * [0] ray-box slab tests, [1] (ray, box) pairs entered, [2] ray-triangle tests of the
+   * algorithm (brute force: every live ray x every triangle; BV: the members of the boxes each
+   * ray entered), [3] ray-triangle tests the wavefronts executed. NULL costs nothing. */
+  uint64_t* d_stats;
+} splat_hitset_t;
+
+/* ---- light-map (splat) parameters (src/ray_imagers.jl:78-165) ---------------------------- */
+#define SPLAT_MAX_LAMBDA 64
+#define SPLAT_MAX_DELTA 8
+#define SPLAT_MAX_CURVE 24
+#define SPLAT_WEIGHTS_IDENTITY 0 /* channel k <- spectrum[k], weight 1 (the reference's tex[i,j,k]) */
+#define SPLAT_WEIGHTS_TABLE 1    /* channel c <- weights[k*3+c] * cos (c < channels <= 3)          */
+#define SPLAT_WEIGHTS_CURVES 2   /* channel c <- curve_c(lambda) * cos, piecewise linear (the retina
+                                    response of src/rgb_spectrum.jl:116-148), lambda = splat lambda */
+
+typedef struct splat_lightmap_params {
+  int32_t n_lambda;  /* grid size; 0 = own-lambda mode (one splat per (dx,dy) at the ray's lambda) */
+  int32_t n_delta;   /* sub-pixel offsets per axis (upres(N), src/alg_types.jl:108)              */
+  int32_t channels;  /* light-map planes                                                         */
+  int32_t weight_mode; /* SPLAT_WEIGHTS_*                                                         */
+  int32_t width, height; /* light-map texels (tex[i,j,c], i fastest, Julia column-major)          */
+  float spectrum[SPLAT_MAX_LAMBDA];
+  float delta[SPLAT_MAX_DELTA];
+  float weights[SPLAT_MAX_LAMBDA * 3]; /* SPLAT_WEIGHTS_TABLE: [k][c] */
+  int32_t curve_n[3];                  /* SPLAT_WEIGHTS_CURVES: points per channel (>= 2) */
+  float curve_x[3][SPLAT_MAX_CURVE];   /* ascending abscissae (nm)                        */
+  float curve_y[3][SPLAT_MAX_CURVE];   /* ordinates (already normalised)                  */
+} splat_lightmap_params_t;
+
+/* ---- pinhole ray source (camera_ray, src/ray_generators.jl:18-62) ------------------------ */
+#define SPLAT_ORDER_COLUMN_MAJOR 0 /* ray k <-> pixel k, rows fastest (the reference's order) */
+#define SPLAT_ORDER_TILED8 1       /* 8x8 pixel tiles, tiles column-major (coherent waves)     */
+#define SPLAT_LAMBDA_FIXED 0       /* every ray at lambda_min (the reference uses 550 nm)       */
+#define SPLAT_LAMBDA_UNIFORM 1     /* per-sample uniform in [lambda_min, lambda_max) from seed  */
+#define SPLAT_LAMBDA_STRATIFIED 2  /* sample l of L: stratum l of [lambda_min, lambda_max)      */
+
+typedef struct splat_pinhole {
+  float pos[3];   /* pinhole (the point light)             */
+  float dir[3];   /* unit view direction                    */
+  float up[3];    /* unit up                                */
+  float right[3]; /* unit right                             */
+  float fov_half_sin;
+  int32_t height; /* rows (x in 1..height)                  */
+  int32_t width;  /* columns (y in 1..width)                */
+  int32_t samples_per_pixel;
+  int32_t order;       /* SPLAT_ORDER_*  */
+  int32_t lambda_mode; /* SPLAT_LAMBDA_* */
+  float lambda_min, lambda_max;
+  float jitter; /* dv scale numerator: dir += dv * jitter / max(h, w) (reference: 0.025) */
+  int32_t _pad;
+  uint64_t seed;
+} splat_pinhole_t;
+
+/* ---- fused bounce loop (run_evolution!, src/tracer.jl:233-282) --------------------------- */
+typedef struct splat_run_params {
+  int32_t depth;         /* bounces                                                    */
+  int32_t first_diffuse; /* 1-based index of the first diffuse triangle                */
+  float force_rand;      /* NaN = seeded per-ray random; 0 = always reflect; 1 = refract */
+  int32_t _pad;
+  uint64_t seed;         /* bounce random numbers: f(seed, bounce, ray.dest)           */
+  splat_optics_t optics;
+  /* Optional instrumentation: 2*(depth+1) hipEvent_t handles (as void*), recorded on the
+   * stream immediately before and after each hitter launch of the loop (bounces 1..depth, then
+   * the realigning pass). NULL records nothing. */
+  void** hit_events;
+  int32_t n_hit_events;
+  int32_t _pad2;
+} splat_run_params_t;
+
+/* ===================================== entry points ====================================== */
+
+const char* splat_strerror(int32_t code);
+int32_t splat_abi_version(void);
+/* Writes sizeof of (adray, tri, ntri, ftri, bvbox, sphere) into out[0..5]. */
+int32_t splat_layout_sizes(int64_t* out6);
+/* Device count visible to HIP (0 when none; never fails). */
+int32_t splat_device_count(void);
+
+/* Triangle preparation: hoists the per-triangle terms of distance_to_plane and in_triangle
+ * (src/geo.jl:433-470) into 64-byte records. Replaces the per-call n_tris tuple build of
+ * trace! (src/tracer.jl:313). */
+size_t splat_hitset_bf_bytes(int32_t n_tris);
+int32_t splat_hitset_prepare_bf(const splat_ntri_t* n_tris, int32_t n_tris_len, void* ws,
+                                size_t ws_bytes, splat_hitset_t* out, void* stream);
+int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t n_tris_len, void* ws,
+                                     size_t ws_bytes, splat_hitset_t* out, void* stream);
+
+/* Bounding-volume preparation from the reference's BV memberships (DPBVHitter constructor,
+ * src/alg_types.jl:81-90, pack_bv_tris src/bv.jl:126-136). `bv_tri_count` (n_bv) and
+ * `bv_tris` (Julia column-major n_bv x bv_stride matrix of 1-based indices) are HOST arrays.
+ * The boxes are rebuilt from the members and padded so the test is conservative. */
+size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members);
+int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t n_tris_len, int32_t n_bv,
+                                const int64_t* bv_tri_count, const int64_t* bv_tris,
+                                int64_t bv_stride, void* ws, size_t ws_bytes,
+                                splat_hitset_t* out, void* stream);
+
+/* next_hit!(tracer, hitter, rays, n_tris) (src/hitters.jl:24-30, :34-82, :168-226, :615-648):
+ * hit_idx[r] = the lexicographic minimum over triangles j of (t_rj, j), ties to the lowest
+ * index, 1 on a miss. `d_count` (device int64, nullable) limits the rays to the first
+ * min(*d_count, R) -- the live count kept on the device between bounces. */
+int32_t splat_next_hit(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
+                       const splat_hitset_t* hs, int32_t* hit_idx, void* stream);
+/* One-call brute force over an n_tris array (prepares into ws, then hits). */
+int32_t splat_next_hit_bf(const splat_adray_t* rays, int64_t R, const splat_ntri_t* n_tris,
+                          int32_t n_tris_len, int32_t* hit_idx, void* ws, size_t ws_bytes,
+                          void* stream);
+
+/* rays .= evolve_ray.(rays, hit_idx, tris[hit_idx], rndm, first_diffuse)
+ * (src/tracer.jl:181-201, :118-179). Random number per ray: force_rand when not NaN
+ * (src/tracer.jl:256-258), else rndm[r] when rndm != NULL, else f(seed, bounce, ray.dest). */
+int32_t splat_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count,
+                     const int32_t* hit_idx, const splat_ftri_t* tris, int32_t n_tris_len,
+                     const float* rndm, float force_rand, uint64_t seed, int32_t bounce,
+                     int32_t first_diffuse, const splat_optics_t* optics, void* stream);
+
+/* partition!(rays, swap; by = r.status != ACTIVE) (src/partition.jl:122-132): stable -- ACTIVE
+ * rays first in their original order, then the rest in their original order -- and in place
+ * (the result is copied back into `rays`). The ACTIVE count goes to d_count_out (device) and,
+ * if h_count_out != NULL, also to the host (this synchronises the stream). */
+size_t splat_partition_ws_bytes(int64_t R);
+int32_t splat_partition_active(splat_adray_t* rays, splat_adray_t* swap, int64_t R,
+                               int64_t* d_count_out, int64_t* h_count_out, void* ws,
+                               size_t ws_bytes, void* stream);
+
+/* continuum_light_map2! (src/ray_imagers.jl:142-165): final_evolution of DIFFUSE rays
+ * (:4-37, mutates rays) then the atomic splat (:78-113) into tex (float, channels x height x
+ * width planes, texel (i,j,c) at i + width*(j + height*c)). tex is accumulated into, not
+ * cleared. */
+int32_t splat_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count,
+                        const int32_t* hit_idx, const splat_ftri_t* tris, int32_t n_tris_len,
+                        const splat_lightmap_params_t* lm, float* tex, void* stream);
+
+/* Seeded pinhole ray source: R = height * width * samples_per_pixel rays. */
+int32_t splat_raygen_pinhole(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R,
+                             void* stream);
+
+/* Fill helper (graph-safe clear, no hipMemsetAsync): n floats <- value. */
+int32_t splat_fill_f32(float* dst, int64_t n, float value, void* stream);
+
+/* run_evolution!(hitter, ::ExperimentalTracer; depth, first_diffuse, ...) fused: every count
+ * stays on the device (no host sync per bounce), so the whole loop can be captured in a HIP
+ * graph. On return `rays` and `hit_idx` hold exactly what the reference's loop plus its final
+ * realigning next_hit! leave (src/tracer.jl:247-281, reorder = false): live rays first, then
+ * the rays retired at bounce depth, ..., then those retired at bounce 1, each group in
+ * order. `d_live_out` (device int64) receives the final live count. */
+size_t splat_run_ws_bytes(int64_t R);
+int32_t splat_run_evolution(splat_adray_t* rays, int64_t R, const splat_hitset_t* hs,
+                            const splat_ftri_t* tris, int32_t n_tris_len, int32_t* hit_idx,
+                            const splat_run_params_t* params, void* ws, size_t ws_bytes,
+                            int64_t* d_live_out, void* stream);
+
+/* One full light-map frame: pinhole rays -> run_evolution -> light map, all on the device.
+ * `tex` is cleared first. `rays`/`hit_idx` must hold cam's R rays; ws from splat_run_ws_bytes. */
+int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R,
+                          const splat_hitset_t* hs, const splat_ftri_t* tris,
+                          int32_t n_tris_len, int32_t* hit_idx, const splat_run_params_t* params,
+                          const splat_lightmap_params_t* lm, float* tex, void* ws,
+                          size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPLAT_HIP_H */

@@ -1,0 +1,233 @@
+"""Host-side mirror of the reference's operator interface for the per-frame hot path.
+
+The reference plugs algorithms together through Julia multiple dispatch on hitter/tracer types
+(src/alg_types.jl): `next_hit!(tracer, hitter, rays, n_tris)` (src/hitters.jl), the
+`evolve_ray` broadcast (src/tracer.jl:181-201), `partition!(rays, swap; by)`
+(src/partition.jl:122-132), `run_evolution!(hitter, tracer; ...)` (src/tracer.jl:233-282) and
+`continuum_light_map2!(; tracer, tris, rays, ...)` (src/ray_imagers.jl:142-165). The same names
+(with `!` spelled `_`), the same argument meanings and in-place semantics are kept here; every
+call goes through the C ABI of libsplat_hip.so. Device arrays are torch tensors holding the
+reference's isbits records as bytes: rays (R, 136) uint8, n_tris (N, 52), tris (N, 120).
+Indices stay 1-based Julia indices. Errors raise SplatError (never a silent CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi, bv as bvmod
+from . import lib as L
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def to_device(a: np.ndarray, device: str = "cuda"):
+    """Structured numpy array -> (n, itemsize) uint8 device tensor (byte-identical)."""
+    torch = _torch()
+    a = np.ascontiguousarray(a)
+    raw = a.view(np.uint8).reshape(a.shape[0], a.dtype.itemsize)
+    return torch.from_numpy(raw.copy()).to(device)
+
+
+def from_device(t, dtype) -> np.ndarray:
+    raw = t.detach().to("cpu").contiguous().numpy()
+    return raw.reshape(-1).view(dtype).copy()
+
+
+def _nrows(t) -> int:
+    return int(t.shape[0])
+
+
+def _check_rows(t, itemsize: int, what: str) -> None:
+    torch = _torch()
+    if t.dtype != torch.uint8 or t.dim() != 2 or t.shape[1] != itemsize or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError(f"{what} must be a contiguous (n, {itemsize}) uint8 CUDA tensor")
+
+
+def _ws(nbytes: int, device="cuda"):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+# ---- hitters (src/alg_types.jl:8-90) --------------------------------------------------------
+
+class HIPBruteForceHitter:
+    """Brute-force hitter: the results of StableHitter and ExperimentalHitter{,3,6}
+    (src/hitters.jl:24-30, :34-82, :168-310) -- lexicographic (t, index) minimum, ties to the
+    lowest index. Holds the prepared 64-byte triangle records as its scratch."""
+
+    def __init__(self, n_tris, tune_waves: int = 0):
+        L.require_gpu()
+        lib = L.lib()
+        self.hs = abi.HitSet()
+        if n_tris.shape[1] == abi.NTRI.itemsize:
+            _check_rows(n_tris, abi.NTRI.itemsize, "n_tris")
+            self.ws = _ws(lib.splat_hitset_bf_bytes(_nrows(n_tris)))
+            L.check(lib.splat_hitset_prepare_bf(L.ptr(n_tris), _nrows(n_tris), L.ptr(self.ws), self.ws.numel(),
+                                                L.ref(self.hs), L.stream_handle()), "prepare_bf")
+        else:
+            _check_rows(n_tris, abi.FTRI.itemsize, "tris")
+            self.ws = _ws(lib.splat_hitset_bf_bytes(_nrows(n_tris)))
+            L.check(lib.splat_hitset_prepare_bf_ftri(L.ptr(n_tris), _nrows(n_tris), L.ptr(self.ws), self.ws.numel(),
+                                                     L.ref(self.hs), L.stream_handle()), "prepare_bf_ftri")
+        self.hs.tune_waves = tune_waves
+
+
+class HIPBVHitter:
+    """One-level bounding-volume hitter: DPBVHitter(A, ray_length, tris, bvs, memberships)
+    (src/alg_types.jl:81-90, src/hitters.jl:615-648). Memberships are packed as pack_bv_tris
+    does (src/bv.jl:126-136); the result equals the brute-force hitter's."""
+
+    def __init__(self, tris, bvs=None, memberships=None, depth: int = 6, host_tris: np.ndarray | None = None):
+        L.require_gpu()
+        _check_rows(tris, abi.FTRI.itemsize, "tris")
+        if memberships is None:
+            if host_tris is None:
+                host_tris = from_device(tris, abi.FTRI)
+            bvs, memberships = bvmod.bv_partition(host_tris, depth)
+        self.bvs = bvs
+        counts, mat, stride = bvmod.pack_bv_tris(memberships, _nrows(tris))
+        self.counts, self.mat, self.stride = counts, np.asfortranarray(mat), stride
+        lib = L.lib()
+        total = int(counts.sum())
+        self.ws = _ws(lib.splat_hitset_bv_bytes(counts.shape[0], total))
+        self.hs = abi.HitSet()
+        L.check(lib.splat_hitset_prepare_bv(L.ptr(tris), _nrows(tris), counts.shape[0],
+                                            counts.ctypes.data_as(C.c_void_p),
+                                            self.mat.ctypes.data_as(C.c_void_p), stride, L.ptr(self.ws),
+                                            self.ws.numel(), L.ref(self.hs), L.stream_handle()), "prepare_bv")
+
+
+class HIPTracer:
+    """ExperimentalTracer(A, rays, N) (src/alg_types.jl:101-115): owns hit_idx (Int32, 1-based),
+    rndm, ray_swap and the sub-pixel offsets delta = upres(N); plus the device workspace of the
+    fused bounce loop."""
+
+    def __init__(self, rays, N: int = 1):
+        torch = _torch()
+        _check_rows(rays, abi.ADRAY.itemsize, "rays")
+        R = _nrows(rays)
+        self.hit_idx = torch.ones(R, dtype=torch.int32, device=rays.device)
+        self.rndm = torch.empty(R, dtype=torch.float32, device=rays.device)
+        self.ray_swap = torch.empty_like(rays)
+        self.delta = abi.upres(N)
+        self.N = N
+        self._run_ws = None
+        self._part_ws = None
+        self.d_count = torch.zeros(1, dtype=torch.int64, device=rays.device)
+
+    def run_ws(self, R: int):
+        if self._run_ws is None or self._run_ws.numel() < L.lib().splat_run_ws_bytes(R):
+            self._run_ws = _ws(L.lib().splat_run_ws_bytes(R))
+        return self._run_ws
+
+    def part_ws(self, R: int):
+        if self._part_ws is None or self._part_ws.numel() < L.lib().splat_partition_ws_bytes(R):
+            self._part_ws = _ws(L.lib().splat_partition_ws_bytes(R))
+        return self._part_ws
+
+
+# ---- operators ----------------------------------------------------------------------------------
+
+def next_hit_(tracer: HIPTracer, hitter, rays, n_tris=None, d_count=None) -> None:
+    """next_hit!(tracer, hitter, rays, n_tris): tracer.hit_idx[r] <- nearest hit index (1 = miss)."""
+    _check_rows(rays, abi.ADRAY.itemsize, "rays")
+    L.check(L.lib().splat_next_hit(L.ptr(rays), _nrows(rays), L.ptr(d_count), L.ref(hitter.hs), L.ptr(tracer.hit_idx),
+                                   L.stream_handle()), "next_hit")
+
+
+def evolve_(rays, hit_idx, tris, *, first_diffuse: int, rndm=None, force_rand=None, seed: int = 0, bounce: int = 1,
+            optics: abi.Optics | None = None, d_count=None) -> None:
+    """rays .= evolve_ray.(rays, hit_idx, tris[hit_idx], rndm, first_diffuse) (src/tracer.jl:227, :261)."""
+    _check_rows(rays, abi.ADRAY.itemsize, "rays")
+    _check_rows(tris, abi.FTRI.itemsize, "tris")
+    opt = optics if optics is not None else abi.optics()
+    fr = math.nan if force_rand is None else float(force_rand)
+    L.check(L.lib().splat_evolve(L.ptr(rays), _nrows(rays), L.ptr(d_count), L.ptr(hit_idx), L.ptr(tris), _nrows(tris),
+                                 L.ptr(rndm), fr, seed, bounce, first_diffuse, L.ref(opt), L.stream_handle()), "evolve")
+
+
+def partition_(vals, swap, tracer: HIPTracer | None = None, host_count: bool = True):
+    """partition!(vals, swap; by = status != ACTIVE) -> number of ACTIVE rays, now first (stable)."""
+    _check_rows(vals, abi.ADRAY.itemsize, "vals")
+    torch = _torch()
+    R = _nrows(vals)
+    ws = tracer.part_ws(R) if tracer is not None else _ws(L.lib().splat_partition_ws_bytes(R))
+    d = tracer.d_count if tracer is not None else torch.zeros(1, dtype=torch.int64, device=vals.device)
+    h = C.c_int64(0)
+    L.check(L.lib().splat_partition_active(L.ptr(vals), L.ptr(swap), R, L.ptr(d), C.byref(h) if host_count else None,
+                                           L.ptr(ws), ws.numel(), L.stream_handle()), "partition")
+    return int(h.value) if host_count else d
+
+
+def run_evolution_(hitter, tracer: HIPTracer, *, depth: int, first_diffuse: int, tris, rays, n_tris=None,
+                   force_rand=None, seed: int = 0, optics: abi.Optics | None = None, reorder: bool = False,
+                   d_live=None) -> None:
+    """run_evolution!(hitter, ::ExperimentalTracer; depth, first_diffuse, n_tris, tris, rays,
+    reorder, force_rand) (src/tracer.jl:233-282), with every count kept on the device."""
+    if reorder:
+        raise NotImplementedError("reorder=true belongs to the backward camera pass, not the light-map path")
+    _check_rows(rays, abi.ADRAY.itemsize, "rays")
+    _check_rows(tris, abi.FTRI.itemsize, "tris")
+    R = _nrows(rays)
+    p = abi.run_params(depth, first_diffuse, math.nan if force_rand is None else float(force_rand), seed, optics)
+    ws = tracer.run_ws(R)
+    L.check(L.lib().splat_run_evolution(L.ptr(rays), R, L.ref(hitter.hs), L.ptr(tris), _nrows(tris),
+                                        L.ptr(tracer.hit_idx), L.ref(p), L.ptr(ws), ws.numel(),
+                                        L.ptr(d_live if d_live is not None else tracer.d_count), L.stream_handle()),
+            "run_evolution")
+
+
+def continuum_light_map2_(*, tracer: HIPTracer, tris, rays, tex, spectrum=None, mode: str = "rgb", lm=None,
+                          d_count=None, **_ignored) -> None:
+    """continuum_light_map2!(; tracer, tris, rays, spectrum, tex, ...) (src/ray_imagers.jl:142-165):
+    final_evolution of DIFFUSE rays, then the atomic splat into tex (C, H, W) float32, which is
+    accumulated into (the reference initialises it with tex_f())."""
+    torch = _torch()
+    _check_rows(rays, abi.ADRAY.itemsize, "rays")
+    if tex.dtype != torch.float32 or tex.dim() != 3 or not tex.is_contiguous():
+        raise ValueError("tex must be a contiguous (channels, height, width) float32 tensor")
+    if lm is None:
+        lm = abi.lightmap_params(int(tex.shape[2]), int(tex.shape[1]), n_delta=tracer.N, spectrum=spectrum, mode=mode)
+    if lm.channels != tex.shape[0] or lm.height != tex.shape[1] or lm.width != tex.shape[2]:
+        raise ValueError("tex shape does not match the light-map parameters")
+    L.check(L.lib().splat_light_map(L.ptr(rays), _nrows(rays), L.ptr(d_count), L.ptr(tracer.hit_idx), L.ptr(tris),
+                                    _nrows(tris), L.ref(lm), L.ptr(tex), L.stream_handle()), "light_map")
+
+
+def rays_from_pinhole(cam: abi.Pinhole, device: str = "cuda"):
+    """Seeded pinhole ray source (camera_ray convention, src/ray_generators.jl:18-62)."""
+    torch = _torch()
+    L.require_gpu()
+    R = cam.height * cam.width * cam.samples_per_pixel
+    rays = torch.empty((R, abi.ADRAY.itemsize), dtype=torch.uint8, device=device)
+    L.check(L.lib().splat_raygen_pinhole(L.ref(cam), L.ptr(rays), R, L.stream_handle()), "raygen")
+    return rays
+
+
+class Frame:
+    """Device-resident state of one light-map frame: rays, tracer arrays, workspace, light map.
+    `trace(...)` runs pinhole -> run_evolution! -> continuum_light_map2! with no host sync."""
+
+    def __init__(self, cam: abi.Pinhole, hitter, tris, params: abi.RunParams, lm: abi.LightMapParams,
+                 device: str = "cuda"):
+        torch = _torch()
+        L.require_gpu()
+        self.cam, self.hitter, self.tris, self.params, self.lm = cam, hitter, tris, params, lm
+        self.R = cam.height * cam.width * cam.samples_per_pixel
+        self.rays = torch.empty((self.R, abi.ADRAY.itemsize), dtype=torch.uint8, device=device)
+        self.hit_idx = torch.empty(self.R, dtype=torch.int32, device=device)
+        self.tex = torch.zeros((lm.channels, lm.height, lm.width), dtype=torch.float32, device=device)
+        self.ws = _ws(L.lib().splat_run_ws_bytes(self.R))
+
+    def trace(self, stream=None) -> None:
+        L.check(L.lib().splat_trace_frame(L.ref(self.cam), L.ptr(self.rays), self.R, L.ref(self.hitter.hs),
+                                          L.ptr(self.tris), _nrows(self.tris), L.ptr(self.hit_idx), L.ref(self.params),
+                                          L.ref(self.lm), L.ptr(self.tex), L.ptr(self.ws), self.ws.numel(),
+                                          L.stream_handle(stream)), "trace_frame")

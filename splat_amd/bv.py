@@ -1,0 +1,58 @@
+"""One-level bounding-volume build on the host: bv_partition / spatial_partition / pack_bv_tris
+(src/bv.jl:68-136).
+
+Semantics kept: median split of triangle centroids on axis `depth % 3`, straddling triangles
+placed in both halves, 2**depth leaves, boxes = member vertex bounds, memberships as 1-based
+indices. Deliberate differences: the leaf order is deterministic (the reference shuffles it with
+an unseeded RNG, src/bv.jl:89), and the degenerate sentinel triangle 1 is left out of every leaf
+(the reference includes it, stretching boxes to the origin; it can never be hit). Neither changes
+any hit result: the BV hitter returns the brute-force answer for any membership that covers the
+triangles.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+
+
+def spatial_partition(verts: np.ndarray, idx: np.ndarray, depth: int) -> list[np.ndarray]:
+    """verts: (N, 3, 3) triangle vertices; idx: array of 0-based slots in this node."""
+    if depth < 0:
+        raise ValueError("negative recursion")
+    if depth == 0 or idx.size == 0:
+        return [idx]
+    dim = depth % 3
+    coords = verts[idx, :, dim]
+    mid = np.median(coords.mean(axis=1))
+    left = idx[coords.min(axis=1) <= mid]
+    right = idx[coords.max(axis=1) >= mid]
+    return spatial_partition(verts, left, depth - 1) + spatial_partition(verts, right, depth - 1)
+
+
+def bv_partition(ftris: np.ndarray, depth: int = 6):
+    """Returns (bvs: BVBOX array, memberships: list of 1-based int64 index arrays)."""
+    verts = ftris["v"][:, 1:4, :].astype(np.float64)
+    slots = np.arange(1, ftris.shape[0], dtype=np.int64)  # skip the degenerate slot 0
+    parts = [p for p in spatial_partition(verts, slots, depth) if p.size > 0]
+    bvs = np.zeros(len(parts), dtype=abi.BVBOX)
+    members = []
+    for k, p in enumerate(parts):
+        v = ftris["v"][p][:, 1:4, :].reshape(-1, 3)
+        bvs[k]["min"] = v.min(axis=0)
+        bvs[k]["max"] = v.max(axis=0)
+        members.append(np.sort(p) + 1)
+    return bvs, members
+
+
+def pack_bv_tris(memberships: list[np.ndarray], n_tris: int, max_overcount_factor: float = 4.0):
+    """pack_bv_tris (src/bv.jl:126-136): dense n_bv x stride matrix of 1-based indices, Julia
+    column-major (element (k, m) at k + m*n_bv), plus the per-BV counts. The stride grows to the
+    largest membership if the reference's ceil(N/n_bv*4) would overflow."""
+    n_bv = len(memberships)
+    counts = np.array([m.size for m in memberships], dtype=np.int64)
+    stride = max(int(np.ceil(n_tris / max(n_bv, 1) * max_overcount_factor)), int(counts.max(initial=1)))
+    mat = np.ones((n_bv, stride), dtype=np.int64, order="F")
+    for k, m in enumerate(memberships):
+        mat[k, : m.size] = np.sort(m)
+    return counts, mat, stride

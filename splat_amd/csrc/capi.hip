@@ -1,0 +1,397 @@
+// capi.hip -- the C ABI (include/splat_hip.h): argument checks, workspace carving, launches and
+// the device-resident bounce loop.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace spl;
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
+
+struct Carver {
+  char* base;
+  size_t used = 0;
+  explicit Carver(void* p) : base(static_cast<char*>(p)) {}
+  template <class T> T* take(size_t count) {
+    T* p = reinterpret_cast<T*>(base ? base + used : nullptr);
+    used += align_up(count * sizeof(T));
+    return p;
+  }
+};
+
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+inline int32_t launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
+}
+
+constexpr int MAX_DEPTH = 64;
+
+struct RunWs {
+  splat_adray_t* s1;
+  splat_adray_t* s2;
+  int32_t* hit_scratch;
+  int32_t* tile_counts;
+  int32_t* tile_offsets;
+  int64_t* counters;  // counters[k] = live rays after bounce k (counters[0] = R)
+  size_t bytes;
+};
+
+RunWs carve_run(void* ws, int64_t R) {
+  Carver c(ws);
+  int64_t n_tiles = (R + TILE - 1) / TILE;
+  RunWs w;
+  w.s1 = c.take<splat_adray_t>((size_t)R);
+  w.s2 = c.take<splat_adray_t>((size_t)R);
+  w.hit_scratch = c.take<int32_t>((size_t)R);
+  w.tile_counts = c.take<int32_t>((size_t)n_tiles);
+  w.tile_offsets = c.take<int32_t>((size_t)n_tiles);
+  w.counters = c.take<int64_t>(MAX_DEPTH + 2);
+  w.bytes = c.used;
+  return w;
+}
+
+struct PartWs {
+  int32_t* tile_counts;
+  int32_t* tile_offsets;
+  int64_t* total;
+  size_t bytes;
+};
+PartWs carve_part(void* ws, int64_t R) {
+  Carver c(ws);
+  int64_t n_tiles = (R + TILE - 1) / TILE;
+  PartWs w;
+  w.tile_counts = c.take<int32_t>((size_t)n_tiles);
+  w.tile_offsets = c.take<int32_t>((size_t)n_tiles);
+  w.total = c.take<int64_t>(1);
+  w.bytes = c.used;
+  return w;
+}
+
+int32_t check_hitset(const splat_hitset_t* hs) {
+  if (!hs || !hs->d_recs || hs->n_tris < 1) return SPLAT_ERR_INVALID_ARG;
+  if (hs->kind == SPLAT_HITTER_BV) {
+    if (!hs->d_boxes || !hs->d_ranges || hs->n_bv < 0) return SPLAT_ERR_INVALID_ARG;
+  } else if (hs->kind != SPLAT_HITTER_BF) {
+    return SPLAT_ERR_INVALID_ARG;
+  }
+  return SPLAT_OK;
+}
+
+void hit_dispatch(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs, int32_t* hit_idx,
+                  hipStream_t s) {
+  if (hs->kind == SPLAT_HITTER_BF) {
+    int w = hs->tune_waves > 0 ? hs->tune_waves : choose_bf_waves(R, hs->n_tris);
+    launch_hit_bf(rays, R, d_count, static_cast<const TriRec*>(hs->d_recs), hs->n_tris, hit_idx, w, hs->d_stats, s);
+  } else {
+    launch_hit_bv(rays, R, d_count, static_cast<const float4*>(hs->d_boxes), static_cast<const int2*>(hs->d_ranges),
+                  static_cast<const TriRec*>(hs->d_recs), hs->n_bv, hit_idx, hs->d_stats, s);
+  }
+}
+
+int32_t check_optics(const splat_optics_t* o) {
+  if (!o) return SPLAT_ERR_INVALID_ARG;
+  if (o->ior_model != SPLAT_IOR_REFERENCE_LINEAR && o->ior_model != SPLAT_IOR_CAUCHY) return SPLAT_ERR_INVALID_ARG;
+  return SPLAT_OK;
+}
+
+int32_t check_lightmap(const splat_lightmap_params_t* lm) {
+  if (!lm || lm->width < 1 || lm->height < 1 || lm->channels < 1) return SPLAT_ERR_INVALID_ARG;
+  if (lm->n_lambda < 0 || lm->n_lambda > SPLAT_MAX_LAMBDA) return SPLAT_ERR_UNSUPPORTED;
+  if (lm->n_delta < 1 || lm->n_delta > SPLAT_MAX_DELTA) return SPLAT_ERR_UNSUPPORTED;
+  switch (lm->weight_mode) {
+    case SPLAT_WEIGHTS_IDENTITY:
+      if (lm->channels != (lm->n_lambda > 0 ? lm->n_lambda : 1)) return SPLAT_ERR_INVALID_ARG;
+      break;
+    case SPLAT_WEIGHTS_TABLE:
+      if (lm->channels > 3 || lm->n_lambda < 1) return SPLAT_ERR_INVALID_ARG;
+      break;
+    case SPLAT_WEIGHTS_CURVES:
+      if (lm->channels > 3) return SPLAT_ERR_INVALID_ARG;
+      for (int c = 0; c < lm->channels; ++c)
+        if (lm->curve_n[c] < 2 || lm->curve_n[c] > SPLAT_MAX_CURVE) return SPLAT_ERR_INVALID_ARG;
+      break;
+    default:
+      return SPLAT_ERR_INVALID_ARG;
+  }
+  return SPLAT_OK;
+}
+
+// the device-resident bounce loop over rays already in w.s1; results land in `final_rays`
+int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris,
+                 int32_t* hit_final, const splat_run_params_t* p, const RunWs& w, hipStream_t s) {
+  const int64_t n_tiles = (R + TILE - 1) / TILE;
+  const bool ev = p->hit_events && p->n_hit_events >= 2 * (p->depth + 1);
+  auto mark = [&](int i) { if (ev) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[i]), s); };
+  launch_set_i64(&w.counters[0], R, s);
+  splat_adray_t* cur = w.s1;
+  splat_adray_t* nxt = w.s2;
+  for (int k = 1; k <= p->depth; ++k) {
+    const int64_t* live = &w.counters[k - 1];
+    mark(2 * (k - 1));
+    hit_dispatch(cur, R, live, hs, w.hit_scratch, s);
+    mark(2 * (k - 1) + 1);
+    launch_evolve(cur, R, live, w.hit_scratch, tris, hs->n_tris, nullptr, p->force_rand, p->seed, k, p->first_diffuse, p->optics,
+                  w.tile_counts, s);
+    launch_scan_tiles(w.tile_counts, n_tiles, live, R, w.tile_offsets, &w.counters[k], s);
+    launch_scatter(cur, R, live, w.tile_offsets, &w.counters[k], nxt, final_rays, w.hit_scratch, hit_final, s);
+    splat_adray_t* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  const int64_t* live = &w.counters[p->depth];
+  mark(2 * p->depth);
+  hit_dispatch(cur, R, live, hs, hit_final, s);  // the reference's realigning next_hit! (src/tracer.jl:280)
+  mark(2 * p->depth + 1);
+  launch_copy_rays(cur, final_rays, R, live, s);
+  return launch_status();
+}
+
+int32_t check_run(const splat_adray_t* rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris, int32_t N,
+                  const int32_t* hit_idx, const splat_run_params_t* p, void* ws, size_t ws_bytes) {
+  if (!rays || R < 0 || !tris || N < 1 || !hit_idx || !p || !ws) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t e = check_hitset(hs)) return e;
+  if (int32_t e = check_optics(&p->optics)) return e;
+  if (p->depth < 0 || p->depth > MAX_DEPTH) return SPLAT_ERR_UNSUPPORTED;
+  if (R > (int64_t)INT32_MAX) return SPLAT_ERR_UNSUPPORTED;
+  if (ws_bytes < carve_run(nullptr, R).bytes) return SPLAT_ERR_WORKSPACE;
+  return SPLAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* splat_strerror(int32_t code) {
+  switch (code) {
+    case SPLAT_OK: return "ok";
+    case SPLAT_ERR_INVALID_ARG: return "invalid argument";
+    case SPLAT_ERR_WORKSPACE: return "workspace too small";
+    case SPLAT_ERR_HIP: return "HIP runtime error";
+    case SPLAT_ERR_UNSUPPORTED: return "unsupported parameter";
+    case SPLAT_ERR_CAPACITY: return "capacity exceeded";
+    case SPLAT_ERR_BUDGET: return "request exceeds the memory budget";
+    default: return "unknown error";
+  }
+}
+
+int32_t splat_abi_version(void) { return SPLAT_ABI_VERSION; }
+
+int32_t splat_layout_sizes(int64_t* out6) {
+  if (!out6) return SPLAT_ERR_INVALID_ARG;
+  out6[0] = sizeof(splat_adray_t);
+  out6[1] = sizeof(splat_tri_t);
+  out6[2] = sizeof(splat_ntri_t);
+  out6[3] = sizeof(splat_ftri_t);
+  out6[4] = sizeof(splat_bvbox_t);
+  out6[5] = sizeof(splat_sphere_t);
+  return SPLAT_OK;
+}
+
+int32_t splat_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+size_t splat_hitset_bf_bytes(int32_t n_tris) { return n_tris > 0 ? align_up((size_t)n_tris * sizeof(TriRec)) : 0; }
+
+int32_t splat_hitset_prepare_bf(const splat_ntri_t* n_tris, int32_t N, void* ws, size_t ws_bytes, splat_hitset_t* out,
+                                void* stream) {
+  if (!n_tris || N < 1 || !ws || !out) return SPLAT_ERR_INVALID_ARG;
+  if (ws_bytes < splat_hitset_bf_bytes(N)) return SPLAT_ERR_WORKSPACE;
+  launch_prep_ntris(n_tris, N, static_cast<TriRec*>(ws), S(stream));
+  std::memset(out, 0, sizeof(*out));
+  out->kind = SPLAT_HITTER_BF;
+  out->n_tris = N;
+  out->d_recs = ws;
+  return launch_status();
+}
+
+int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t N, void* ws, size_t ws_bytes, splat_hitset_t* out,
+                                     void* stream) {
+  if (!tris || N < 1 || !ws || !out) return SPLAT_ERR_INVALID_ARG;
+  if (ws_bytes < splat_hitset_bf_bytes(N)) return SPLAT_ERR_WORKSPACE;
+  launch_prep_ftris(tris, N, static_cast<TriRec*>(ws), S(stream));
+  std::memset(out, 0, sizeof(*out));
+  out->kind = SPLAT_HITTER_BF;
+  out->n_tris = N;
+  out->d_recs = ws;
+  return launch_status();
+}
+
+size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members) {
+  if (n_bv < 1 || total_members < 0) return 0;
+  return align_up((size_t)total_members * sizeof(TriRec)) + align_up((size_t)n_bv * 2 * sizeof(float4)) +
+         align_up((size_t)n_bv * sizeof(int2)) + align_up((size_t)(total_members > 0 ? total_members : 1) * sizeof(int32_t));
+}
+
+int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_bv, const int64_t* bv_tri_count,
+                                const int64_t* bv_tris, int64_t bv_stride, void* ws, size_t ws_bytes, splat_hitset_t* out,
+                                void* stream) {
+  if (!tris || N < 1 || n_bv < 1 || !bv_tri_count || !bv_tris || bv_stride < 0 || !ws || !out) return SPLAT_ERR_INVALID_ARG;
+  // flatten the Julia column-major (n_bv x stride) membership matrix; drop the sentinel index 1
+  std::vector<int32_t> members;
+  std::vector<int2> ranges((size_t)n_bv);
+  int32_t max_members = 0;
+  for (int32_t b = 0; b < n_bv; ++b) {
+    int64_t cnt = bv_tri_count[b];
+    if (cnt < 0 || cnt > bv_stride) return SPLAT_ERR_INVALID_ARG;
+    int32_t start = (int32_t)members.size();
+    for (int64_t m = 0; m < cnt; ++m) {
+      int64_t idx = bv_tris[b + m * (int64_t)n_bv];
+      if (idx < 1 || idx > N) return SPLAT_ERR_INVALID_ARG;
+      if (idx == 1) continue;
+      members.push_back((int32_t)idx);
+    }
+    int32_t c = (int32_t)members.size() - start;
+    ranges[(size_t)b] = make_int2(start, c);
+    max_members = c > max_members ? c : max_members;
+  }
+  const int64_t total = (int64_t)members.size();
+  if (ws_bytes < splat_hitset_bv_bytes(n_bv, total)) return SPLAT_ERR_WORKSPACE;
+  Carver c(ws);
+  TriRec* recs = c.take<TriRec>((size_t)total);
+  float4* boxes = c.take<float4>((size_t)n_bv * 2);
+  int2* d_ranges = c.take<int2>((size_t)n_bv);
+  int32_t* d_members = c.take<int32_t>((size_t)(total > 0 ? total : 1));
+  hipStream_t s = S(stream);
+  if (total > 0 && hipMemcpyAsync(d_members, members.data(), (size_t)total * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess)
+    return SPLAT_ERR_HIP;
+  if (hipMemcpyAsync(d_ranges, ranges.data(), ranges.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess)
+    return SPLAT_ERR_HIP;
+  launch_prep_members(tris, d_members, total, recs, s);
+  launch_bv_boxes(tris, d_members, d_ranges, n_bv, boxes, s);
+  if (int32_t e = launch_status()) return e;
+  // the host vectors must outlive the copies
+  if (hipStreamSynchronize(s) != hipSuccess) return SPLAT_ERR_HIP;
+  std::memset(out, 0, sizeof(*out));
+  out->kind = SPLAT_HITTER_BV;
+  out->n_tris = N;
+  out->n_bv = n_bv;
+  out->max_members = max_members;
+  out->total_members = total;
+  out->d_recs = recs;
+  out->d_boxes = boxes;
+  out->d_ranges = d_ranges;
+  return SPLAT_OK;
+}
+
+int32_t splat_next_hit(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs,
+                       int32_t* hit_idx, void* stream) {
+  if (R < 0 || (R > 0 && (!rays || !hit_idx))) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t e = check_hitset(hs)) return e;
+  hit_dispatch(rays, R, d_count, hs, hit_idx, S(stream));
+  return launch_status();
+}
+
+int32_t splat_next_hit_bf(const splat_adray_t* rays, int64_t R, const splat_ntri_t* n_tris, int32_t N, int32_t* hit_idx,
+                          void* ws, size_t ws_bytes, void* stream) {
+  splat_hitset_t hs;
+  if (int32_t e = splat_hitset_prepare_bf(n_tris, N, ws, ws_bytes, &hs, stream)) return e;
+  return splat_next_hit(rays, R, nullptr, &hs, hit_idx, stream);
+}
+
+int32_t splat_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
+                     const splat_ftri_t* tris, int32_t N, const float* rndm, float force_rand, uint64_t seed,
+                     int32_t bounce, int32_t first_diffuse, const splat_optics_t* optics, void* stream) {
+  if (R < 0 || (R > 0 && (!rays || !hit_idx)) || !tris || N < 1) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t e = check_optics(optics)) return e;
+  launch_evolve(rays, R, d_count, hit_idx, tris, N, rndm, force_rand, seed, bounce, first_diffuse, *optics, nullptr, S(stream));
+  return launch_status();
+}
+
+size_t splat_partition_ws_bytes(int64_t R) { return R >= 0 ? carve_part(nullptr, R).bytes : 0; }
+
+int32_t splat_partition_active(splat_adray_t* rays, splat_adray_t* swap, int64_t R, int64_t* d_count_out,
+                               int64_t* h_count_out, void* ws, size_t ws_bytes, void* stream) {
+  if (R < 0 || (R > 0 && (!rays || !swap)) || !ws) return SPLAT_ERR_INVALID_ARG;
+  if (R > (int64_t)INT32_MAX) return SPLAT_ERR_UNSUPPORTED;
+  if (ws_bytes < splat_partition_ws_bytes(R)) return SPLAT_ERR_WORKSPACE;
+  PartWs w = carve_part(ws, R);
+  hipStream_t s = S(stream);
+  int64_t* total = d_count_out ? d_count_out : w.total;
+  const int64_t n_tiles = (R + TILE - 1) / TILE;
+  if (R == 0) {
+    launch_set_i64(total, 0, s);
+  } else {
+    launch_count_active(rays, R, nullptr, w.tile_counts, s);
+    launch_scan_tiles(w.tile_counts, n_tiles, nullptr, R, w.tile_offsets, total, s);
+    launch_scatter(rays, R, nullptr, w.tile_offsets, total, swap, swap, nullptr, nullptr, s);
+    if (hipMemcpyAsync(rays, swap, (size_t)R * sizeof(splat_adray_t), hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return SPLAT_ERR_HIP;
+  }
+  if (int32_t e = launch_status()) return e;
+  if (h_count_out) {
+    if (hipMemcpyAsync(h_count_out, total, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess) return SPLAT_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return SPLAT_ERR_HIP;
+  }
+  return SPLAT_OK;
+}
+
+int32_t splat_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
+                        const splat_ftri_t* tris, int32_t N, const splat_lightmap_params_t* lm, float* tex, void* stream) {
+  if (R < 0 || (R > 0 && (!rays || !hit_idx)) || !tris || N < 1 || !tex) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t e = check_lightmap(lm)) return e;
+  launch_light_map(rays, R, d_count, hit_idx, tris, N, *lm, tex, S(stream));
+  return launch_status();
+}
+
+int32_t splat_raygen_pinhole(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R, void* stream) {
+  if (!cam || !rays || cam->height < 1 || cam->width < 1 || cam->samples_per_pixel < 1) return SPLAT_ERR_INVALID_ARG;
+  if ((int64_t)cam->height * cam->width * cam->samples_per_pixel != R) return SPLAT_ERR_INVALID_ARG;
+  if (!(cam->fov_half_sin > 0.f)) return SPLAT_ERR_INVALID_ARG;
+  launch_raygen(*cam, rays, R, S(stream));
+  return launch_status();
+}
+
+int32_t splat_fill_f32(float* dst, int64_t n, float value, void* stream) {
+  if (n < 0 || (n > 0 && !dst)) return SPLAT_ERR_INVALID_ARG;
+  launch_fill_f32(dst, n, value, S(stream));
+  return launch_status();
+}
+
+size_t splat_run_ws_bytes(int64_t R) { return R >= 0 ? carve_run(nullptr, R).bytes : 0; }
+
+int32_t splat_run_evolution(splat_adray_t* rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris, int32_t N,
+                            int32_t* hit_idx, const splat_run_params_t* p, void* ws, size_t ws_bytes, int64_t* d_live_out,
+                            void* stream) {
+  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes)) return e;
+  RunWs w = carve_run(ws, R);
+  hipStream_t s = S(stream);
+  if (R > 0 && hipMemcpyAsync(w.s1, rays, (size_t)R * sizeof(splat_adray_t), hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return SPLAT_ERR_HIP;
+  if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s)) return e;
+  if (d_live_out &&
+      hipMemcpyAsync(d_live_out, &w.counters[p->depth], sizeof(int64_t), hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return SPLAT_ERR_HIP;
+  return SPLAT_OK;
+}
+
+int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R, const splat_hitset_t* hs,
+                          const splat_ftri_t* tris, int32_t N, int32_t* hit_idx, const splat_run_params_t* p,
+                          const splat_lightmap_params_t* lm, float* tex, void* ws, size_t ws_bytes, void* stream) {
+  if (!cam || !tex) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes)) return e;
+  if (int32_t e = check_lightmap(lm)) return e;
+  if ((int64_t)cam->height * cam->width * cam->samples_per_pixel != R || !(cam->fov_half_sin > 0.f))
+    return SPLAT_ERR_INVALID_ARG;
+  RunWs w = carve_run(ws, R);
+  hipStream_t s = S(stream);
+  launch_fill_f32(tex, (int64_t)lm->width * lm->height * lm->channels, 0.0f, s);
+  launch_raygen(*cam, w.s1, R, s);
+  if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s)) return e;
+  launch_light_map(rays, R, nullptr, hit_idx, tris, N, *lm, tex, s);
+  return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// hit.hip -- ray/triangle hitters for gfx950.
+//
+// Design (replaces EH1-EH6 and DPBVHitter, src/hitters.jl:34-648):
+//  * one lane = one ray, kept in VGPRs for the whole scan; a wave is 64 rays;
+//  * the triangle stream is wave-uniform, so each 64-byte prepared record arrives through the
+//    scalar unit (s_load_dwordx16) into SGPRs and feeds all 64 lanes as an instruction operand:
+//    no LDS traffic and no cross-lane shuffles per test (the reference rotates rays through
+//    32-lane shuffles, src/hitters.jl:111-130);
+//  * brute force splits the triangle list across W waves of a workgroup that share the same 64
+//    rays, then reduces the W packed (t, index) keys per ray through LDS -- deterministic, no
+//    global atomics, ties to the lowest index;
+//  * the BV hitter walks the conservative member boxes per wave; a box whose slab test fails for
+//    all 64 lanes (one ballot) is skipped, otherwise its member records are scanned.
+#include "internal.hpp"
+
+namespace spl {
+
+// ---- preparation --------------------------------------------------------------------------
+__global__ void k_prep_ntris(const splat_ntri_t* __restrict__ tris, int32_t N, TriRec* __restrict__ out) {
+  int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const splat_ntri_t& t = tris[j];
+  out[j] = tri_prep(ld3(t.v[0]), ld3(t.v[1]), ld3(t.v[2]), ld3(t.v[3]), t.idx);
+}
+
+__global__ void k_prep_ftris(const splat_ftri_t* __restrict__ tris, int32_t N, TriRec* __restrict__ out) {
+  int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const splat_ftri_t& t = tris[j];
+  out[j] = tri_prep(ld3(t.v[0]), ld3(t.v[1]), ld3(t.v[2]), ld3(t.v[3]), j + 1);
+}
+
+// members: 1-based triangle indices, concatenated per BV
+__global__ void k_prep_members(const splat_ftri_t* __restrict__ tris, const int32_t* __restrict__ members,
+                               int64_t total, TriRec* __restrict__ out) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= total) return;
+  int32_t idx = members[m];
+  const splat_ftri_t& t = tris[idx - 1];
+  out[m] = tri_prep(ld3(t.v[0]), ld3(t.v[1]), ld3(t.v[2]), ld3(t.v[3]), idx);
+}
+
+// one workgroup per BV: member vertex bounds, padded (relative 1e-3 of the extent, 1e-5 of the
+// magnitude, 1e-6 absolute) so that rounding in the slab test can never cull a hit.
+__global__ __launch_bounds__(256) void k_bv_boxes(const splat_ftri_t* __restrict__ tris, const int32_t* __restrict__ members,
+                                                  const int2* __restrict__ ranges, float4* __restrict__ boxes) {
+  __shared__ float lo_s[3][256], hi_s[3][256];
+  const int b = blockIdx.x;
+  const int2 rg = ranges[b];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int m = threadIdx.x; m < rg.y; m += blockDim.x) {
+    const splat_ftri_t& t = tris[members[rg.x + m] - 1];
+    for (int v = 1; v <= 3; ++v)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(lo[k], t.v[v][k]);
+        hi[k] = fmaxf(hi[k], t.v[v][k]);
+      }
+  }
+  for (int k = 0; k < 3; ++k) { lo_s[k][threadIdx.x] = lo[k]; hi_s[k][threadIdx.x] = hi[k]; }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) {
+        lo_s[k][threadIdx.x] = fminf(lo_s[k][threadIdx.x], lo_s[k][threadIdx.x + s]);
+        hi_s[k][threadIdx.x] = fmaxf(hi_s[k][threadIdx.x], hi_s[k][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float ext = 0.f, mag = 0.f;
+    for (int k = 0; k < 3; ++k) {
+      ext = fmaxf(ext, hi_s[k][0] - lo_s[k][0]);
+      mag = fmaxf(mag, fmaxf(fabsf(lo_s[k][0]), fabsf(hi_s[k][0])));
+    }
+    float pad = 1e-3f * ext + 1e-5f * mag + 1e-6f;
+    if (rg.y == 0) {  // empty BV: a box no ray can enter
+      boxes[2 * b] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+      boxes[2 * b + 1] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    } else {
+      boxes[2 * b] = make_float4(lo_s[0][0] - pad, lo_s[1][0] - pad, lo_s[2][0] - pad, 0.f);
+      boxes[2 * b + 1] = make_float4(hi_s[0][0] + pad, hi_s[1][0] + pad, hi_s[2][0] + pad, 0.f);
+    }
+  }
+}
+
+// ---- ray loading ----------------------------------------------------------------------------
+struct RayLanes {
+  float ox, oy, oz, dx, dy, dz;
+  int64_t ig;
+};
+__device__ __forceinline__ RayLanes load_ray_lanes(const splat_adray_t* __restrict__ rays, int64_t r, bool valid) {
+  RayLanes L{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+  if (valid) {
+    const splat_adray_t* p = rays + r;
+    L.ox = p->pos[0]; L.oy = p->pos[1]; L.oz = p->pos[2];
+    L.dx = p->dir[0]; L.dy = p->dir[1]; L.dz = p->dir[2];
+    L.ig = p->ignore_tri;
+  }
+  return L;
+}
+
+__device__ __forceinline__ int64_t live_count(int64_t R, const int64_t* __restrict__ d_count) {
+  if (!d_count) return R;
+  int64_t c = *d_count;
+  return c < R ? c : R;
+}
+
+// ---- brute force ------------------------------------------------------------------------------
+// Workgroup = W waves sharing 64 rays; wave w scans record slots 1+w, 1+w+W, ... (slot 0 is the
+// degenerate sentinel). Within a wave indices increase, so a strict t < best keeps the lowest
+// index on ties; across waves the packed-key minimum does.
+template <int W, bool STATS>
+__global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restrict__ rays, int64_t R,
+                                                   const int64_t* __restrict__ d_count,
+                                                   const TriRec* __restrict__ recs, int32_t N,
+                                                   int32_t* __restrict__ hit_idx, unsigned long long* stats) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t Rl = live_count(R, d_count);
+  const int64_t base = (int64_t)blockIdx.x * 64;
+  if (base >= Rl) return;  // uniform over the workgroup
+  const int64_t r = base + lane;
+  const bool valid = r < Rl;
+  RayLanes L = load_ray_lanes(rays, r, valid);
+  float best = INFINITY;
+  int32_t bi = 1;
+#pragma unroll 4
+  for (int32_t j = 1 + wave; j < N; j += W) {
+    const TriRec& T = recs[j];
+    float t;
+    bool h = hit_rec(T.q0, T.q1, T.q2, T.q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
+    if (h && t < best) {
+      best = t;
+      bi = __float_as_int(T.q1.w);
+    }
+  }
+  if constexpr (STATS) {
+    const unsigned long long nv = (unsigned long long)__builtin_popcountll(__builtin_amdgcn_ballot_w64(valid));
+    if (lane == 0) {
+      const unsigned long long scanned = N > 1 + wave ? (unsigned long long)((N - 2 - wave) / W + 1) : 0ull;
+      atomicAdd(stats + 2, nv * scanned);
+      atomicAdd(stats + 3, 64ull * scanned);
+    }
+  }
+  if constexpr (W == 1) {
+    if (valid) hit_idx[r] = bi;
+  } else {
+    __shared__ uint64_t keys[W][64];
+    keys[wave][lane] = pack_key(best, bi);
+    __syncthreads();
+    if (wave == 0) {
+      uint64_t k = keys[0][lane];
+#pragma unroll
+      for (int w = 1; w < W; ++w) {
+        uint64_t o = keys[w][lane];
+        k = o < k ? o : k;
+      }
+      if (valid) hit_idx[r] = key_index(k);
+    }
+  }
+}
+
+// ---- one-level bounding volumes ------------------------------------------------------------
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
+                                                const int64_t* __restrict__ d_count,
+                                                const float4* __restrict__ boxes, const int2* __restrict__ ranges,
+                                                const TriRec* __restrict__ recs, int32_t n_bv,
+                                                int32_t* __restrict__ hit_idx, unsigned long long* stats) {
+  const int64_t Rl = live_count(R, d_count);
+  const int64_t wave_base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
+  if (wave_base >= Rl) return;  // uniform over the wave
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool valid = r < Rl;
+  RayLanes L = load_ray_lanes(rays, r, valid);
+  // slab-test reciprocals; |d| < 1e-30 is clamped so 0 * inf never appears
+  auto safe_inv = [](float d) {
+    float a = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
+    return 1.0f / a;
+  };
+  const float ix = safe_inv(L.dx), iy = safe_inv(L.dy), iz = safe_inv(L.dz);
+  float best = INFINITY;
+  int32_t bi = 1;
+  unsigned long long n_enter = 0, n_alg = 0, n_exec = 0;
+  for (int32_t b = 0; b < n_bv; ++b) {
+    const float4 lo = boxes[2 * b], hi = boxes[2 * b + 1];
+    float t1 = (lo.x - L.ox) * ix, t2 = (hi.x - L.ox) * ix;
+    float tmin = fmaxf(0.0f, fminf(t1, t2)), tmax = fmaxf(t1, t2);
+    t1 = (lo.y - L.oy) * iy; t2 = (hi.y - L.oy) * iy;
+    tmin = fmaxf(tmin, fminf(t1, t2)); tmax = fminf(tmax, fmaxf(t1, t2));
+    t1 = (lo.z - L.oz) * iz; t2 = (hi.z - L.oz) * iz;
+    tmin = fmaxf(tmin, fminf(t1, t2)); tmax = fminf(tmax, fmaxf(t1, t2));
+    const bool enter = valid && (tmin <= tmax) && (tmin <= best);
+    if (__builtin_amdgcn_ballot_w64(enter) == 0) continue;
+    const int2 rg = ranges[b];
+    if constexpr (STATS) {
+      n_enter += enter ? 1ull : 0ull;
+      n_alg += enter ? (unsigned long long)rg.y : 0ull;
+      n_exec += (unsigned long long)rg.y;
+    }
+#pragma unroll 4
+    for (int32_t m = 0; m < rg.y; ++m) {
+      const TriRec& T = recs[rg.x + m];
+      float t;
+      bool h = hit_rec(T.q0, T.q1, T.q2, T.q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
+      int32_t idx = __float_as_int(T.q1.w);
+      if (h && (t < best || (t == best && idx < bi))) {
+        best = t;
+        bi = idx;
+      }
+    }
+  }
+  if (valid) hit_idx[r] = bi;
+  if constexpr (STATS) {
+    if (valid) {
+      atomicAdd(stats + 0, (unsigned long long)n_bv);
+      atomicAdd(stats + 1, n_enter);
+      atomicAdd(stats + 2, n_alg);
+    }
+    if ((threadIdx.x & 63) == 0) atomicAdd(stats + 3, 64ull * n_exec);
+  }
+}
+
+// ---- launchers -------------------------------------------------------------------------------
+void launch_prep_ntris(const splat_ntri_t* tris, int32_t N, TriRec* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep_ntris, dim3((N + 255) / 256), dim3(256), 0, s, tris, N, out);
+}
+void launch_prep_ftris(const splat_ftri_t* tris, int32_t N, TriRec* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_prep_ftris, dim3((N + 255) / 256), dim3(256), 0, s, tris, N, out);
+}
+void launch_prep_members(const splat_ftri_t* tris, const int32_t* members, int64_t total, TriRec* out, hipStream_t s) {
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_prep_members, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, tris, members, total, out);
+}
+void launch_bv_boxes(const splat_ftri_t* tris, const int32_t* members, const int2* ranges, int32_t n_bv, float4* boxes,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_bv_boxes, dim3(n_bv), dim3(256), 0, s, tris, members, ranges, boxes);
+}
+
+int choose_bf_waves(int64_t R, int32_t N) {
+  // enough waves to fill 256 CUs x 4 SIMDs several times over, but no more triangle splitting
+  // than the triangle count justifies
+  int64_t groups = (R + 63) / 64;
+  int w = 1;
+  while (w < 16 && groups * w < 8 * 1024 && (N / (2 * w)) >= 32) w *= 2;
+  return w;
+}
+
+template <bool ST>
+static void launch_hit_bf_t(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
+                            int32_t* hit_idx, int waves, unsigned long long* st, hipStream_t s) {
+  dim3 grid((unsigned)((R + 63) / 64));
+  switch (waves) {
+    case 1: hipLaunchKernelGGL((k_hit_bf<1, ST>), grid, dim3(64), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
+    case 2: hipLaunchKernelGGL((k_hit_bf<2, ST>), grid, dim3(128), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
+    case 4: hipLaunchKernelGGL((k_hit_bf<4, ST>), grid, dim3(256), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
+    case 8: hipLaunchKernelGGL((k_hit_bf<8, ST>), grid, dim3(512), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
+    default: hipLaunchKernelGGL((k_hit_bf<16, ST>), grid, dim3(1024), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
+  }
+}
+
+void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
+                   int32_t* hit_idx, int waves, uint64_t* stats, hipStream_t s) {
+  if (R <= 0) return;
+  auto* st = reinterpret_cast<unsigned long long*>(stats);
+  if (st) launch_hit_bf_t<true>(rays, R, d_count, recs, N, hit_idx, waves, st, s);
+  else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, waves, st, s);
+}
+
+void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const float4* boxes, const int2* ranges,
+                   const TriRec* recs, int32_t n_bv, int32_t* hit_idx, uint64_t* stats, hipStream_t s) {
+  if (R <= 0) return;
+  auto* st = reinterpret_cast<unsigned long long*>(stats);
+  dim3 grid((unsigned)((R + 255) / 256));
+  if (st)
+    hipLaunchKernelGGL(k_hit_bv<true>, grid, dim3(256), 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, hit_idx, st);
+  else
+    hipLaunchKernelGGL(k_hit_bv<false>, grid, dim3(256), 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, hit_idx, st);
+}
+
+}  // namespace spl

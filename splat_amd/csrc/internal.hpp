@@ -1,0 +1,46 @@
+// internal.hpp -- launchers shared between the kernel files and the C ABI (not installed).
+#pragma once
+
+#include "spec.hpp"
+
+namespace spl {
+
+constexpr int TILE = 256;  // rays per partition tile / evolve workgroup
+
+// hit.hip
+void launch_prep_ntris(const splat_ntri_t* tris, int32_t N, TriRec* out, hipStream_t s);
+void launch_prep_ftris(const splat_ftri_t* tris, int32_t N, TriRec* out, hipStream_t s);
+void launch_prep_members(const splat_ftri_t* tris, const int32_t* members, int64_t total, TriRec* out, hipStream_t s);
+void launch_bv_boxes(const splat_ftri_t* tris, const int32_t* members, const int2* ranges, int32_t n_bv, float4* boxes,
+                     hipStream_t s);
+int choose_bf_waves(int64_t R, int32_t N);
+void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
+                   int32_t* hit_idx, int waves, uint64_t* stats, hipStream_t s);
+void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const float4* boxes, const int2* ranges,
+                   const TriRec* recs, int32_t n_bv, int32_t* hit_idx, uint64_t* stats, hipStream_t s);
+
+// trace.hip
+void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
+                   const splat_ftri_t* tris, int32_t N, const float* rndm, float force_rand, uint64_t seed, int32_t bounce,
+                   int32_t first_diffuse, const splat_optics_t& opt, int32_t* tile_counts, hipStream_t s);
+void launch_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
+                      const splat_ftri_t* tris, int32_t N, const splat_lightmap_params_t& lm, float* tex,
+                      hipStream_t s);
+void launch_raygen(const splat_pinhole_t& cam, splat_adray_t* rays, int64_t R, hipStream_t s);
+void launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s);
+void launch_set_i64(int64_t* dst, int64_t v, hipStream_t s);
+void launch_copy_rays(const splat_adray_t* src, splat_adray_t* dst, int64_t R, const int64_t* d_count, hipStream_t s);
+
+// partition.hip
+void launch_count_active(const splat_adray_t* rays, int64_t R, const int64_t* d_count, int32_t* tile_counts,
+                         hipStream_t s);
+// exclusive scan of tile counts -> tile offsets; total -> *d_total
+void launch_scan_tiles(const int32_t* tile_counts, int64_t n_tiles, const int64_t* d_count, int64_t R,
+                       int32_t* tile_offsets, int64_t* d_total, hipStream_t s);
+// stable scatter: ACTIVE -> out_active[offset + rank], others -> out_rest[*d_total + rest_rank];
+// the optional carry array moves with the rest part (carry_rest aligned with out_rest).
+void launch_scatter(const splat_adray_t* in, int64_t R, const int64_t* d_count, const int32_t* tile_offsets,
+                    const int64_t* d_total, splat_adray_t* out_active, splat_adray_t* out_rest, const int32_t* carry_in,
+                    int32_t* carry_rest, hipStream_t s);
+
+}  // namespace spl

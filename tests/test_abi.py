@@ -1,0 +1,79 @@
+"""The C-ABI library loads and exports every entry point include/splat_hip.h declares, and the
+Python mirrors of its structs have the C sizes (no GPU compute needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from splat_amd import abi
+from splat_amd import lib as L
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "splat_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(splat_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_wrapper_covers_the_header():
+    assert set(declared_functions()) <= set(L.SIGNATURES)
+
+
+def test_layout_sizes_match_julia_isbits():
+    out = (C.c_int64 * 6)()
+    assert L.lib().splat_layout_sizes(C.cast(out, C.c_void_p)) == 0
+    assert list(out) == [136, 48, 52, 120, 24, 16]
+    assert [abi.ADRAY.itemsize, abi.NTRI.itemsize, abi.FTRI.itemsize, abi.BVBOX.itemsize, abi.SPHERE.itemsize] == \
+        [136, 52, 120, 24, 16]
+
+
+def test_strerror_and_version():
+    lib = L.lib()
+    assert lib.splat_abi_version() == 1
+    assert lib.splat_strerror(0) == b"ok"
+    assert lib.splat_strerror(-2) == b"workspace too small"
+
+
+def test_workspace_sizes_are_monotone():
+    lib = L.lib()
+    assert lib.splat_run_ws_bytes(1000) >= 1000 * (2 * 136 + 4)
+    assert lib.splat_run_ws_bytes(2000) > lib.splat_run_ws_bytes(1000)
+    assert lib.splat_partition_ws_bytes(0) > 0
+    assert lib.splat_hitset_bf_bytes(10) >= 640
+    assert lib.splat_hitset_bv_bytes(4, 100) >= 100 * 64 + 4 * 32
+
+
+def test_argument_errors_without_a_gpu():
+    """Invalid arguments are rejected before anything is launched."""
+    lib = L.lib()
+    assert lib.splat_next_hit(None, -1, None, None, None, None) == -1
+    assert lib.splat_partition_active(None, None, -5, None, None, None, 0, None) == -1
+    hs = abi.HitSet()
+    assert lib.splat_next_hit(None, 0, None, L.ref(hs), None, None) == -1  # no records
+    p = abi.run_params(65, 2)
+    assert lib.splat_run_evolution(C.c_void_p(8), 10, L.ref(hs), C.c_void_p(8), 3, C.c_void_p(8), L.ref(p),
+                                   C.c_void_p(8), 1 << 30, None, None) == -1
+    lm = abi.lightmap_params(8, 8)
+    lm.n_delta = 99
+    assert lib.splat_light_map(C.c_void_p(8), 1, None, C.c_void_p(8), C.c_void_p(8), 3, L.ref(lm), C.c_void_p(8),
+                               None) == -4
+
+
+def test_retina_curves_normalised():
+    for xs, ys in abi.retina_curves():
+        assert xs[0] == 399 and xs[-1] == 700 and np.all(np.diff(xs) > 0)
+        assert ys.max() > 0
+    d = abi.upres(4)
+    assert np.allclose(d, [-0.375, -0.125, 0.125, 0.375])
+    assert abi.upres(1)[0] == 0.0

@@ -1,0 +1,315 @@
+"""HIP path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bars: hit indices, partition permutations and ray states bit-exact against the FP32 oracle;
+light maps within 1e-4 relative L2 of the FP32 oracle (only the atomic accumulation order
+differs); distances and tangents within 1e-5 relative of the FP64 oracle on well-conditioned rays.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from splat_amd import abi, api, bv, scene
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    sc = scene.scene_c1()
+    cam = scene.camera_c1(seed=11, height=64, width=64)
+    rays = O.raygen(cam)
+    return sc, cam, rays
+
+
+# ---- ray source ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("order,lmode,spp,jitter", [
+    (abi.ORDER_TILED8, abi.LAMBDA_UNIFORM, 1, 0.0),
+    (abi.ORDER_COLUMN_MAJOR, abi.LAMBDA_FIXED, 1, 0.025),
+    (abi.ORDER_TILED8, abi.LAMBDA_STRATIFIED, 4, 0.025),
+    (abi.ORDER_TILED8, abi.LAMBDA_UNIFORM, 1, 0.025),  # 36 x 20 is not a multiple of 8: column-major fallback
+])
+def test_raygen_bitexact(order, lmode, spp, jitter):
+    h, w = (36, 20) if jitter and lmode == abi.LAMBDA_UNIFORM else (32, 48)
+    cam = scene.camera_c1(seed=5, height=h, width=w, spp=spp, order=order, lambda_mode=lmode, jitter=jitter)
+    want = O.raygen(cam)
+    got = api.from_device(api.rays_from_pinhole(cam), abi.ADRAY)
+    assert got.tobytes() == want.tobytes()
+
+
+# ---- hitters ------------------------------------------------------------------------------
+
+def _bf_hits(rays_np, ntris_np, waves=0):
+    import torch
+
+    rays = api.to_device(rays_np)
+    hitter = api.HIPBruteForceHitter(api.to_device(ntris_np), tune_waves=waves)
+    tr = api.HIPTracer(rays)
+    api.next_hit_(tr, hitter, rays)
+    _sync()
+    return tr.hit_idx.cpu().numpy()
+
+
+@pytest.mark.parametrize("waves", [0, 1, 2, 4, 8, 16])
+def test_next_hit_bf_bitexact(c1, waves):
+    sc, cam, rays = c1
+    want, _ = O.next_hit(rays, sc.ntris())
+    got = _bf_hits(rays, sc.ntris(), waves)
+    assert (got == want).all()
+    assert (want >= sc.first_diffuse).sum() > 100 and ((want > 1) & (want < sc.first_diffuse)).sum() > 100
+
+
+def test_next_hit_bf_random_rays_and_ignore(c1):
+    sc = c1[0]
+    pts = H.sample_surface_points(sc.ftris, 3000, seed=3)
+    rays = np.concatenate([H.rays_towards(pts, seed=4), H.random_rays(2000, seed=5, n_tris=sc.n_tris)])
+    # rays that start on a triangle and must ignore it
+    rays[:500]["pos"] = pts[:500].astype(np.float32)
+    want, _ = O.next_hit(rays, sc.ntris())
+    rays[:500]["ignore_tri"] = want[:500]
+    want, _ = O.next_hit(rays, sc.ntris())
+    got = _bf_hits(rays, sc.ntris())
+    assert (got == want).all()
+
+
+def test_next_hit_ties_lowest_index():
+    base = scene.sphere_mesh(1)
+    sc = scene.assemble([base, base, base], [], "triplicate")  # every triangle three times
+    pts = H.sample_surface_points(sc.ftris[: 1 + base.shape[0]], 1500, seed=8)
+    rays = H.rays_towards(pts, seed=9)
+    want, _ = O.next_hit(rays, sc.ntris())
+    got = _bf_hits(rays, sc.ntris())
+    assert (got == want).all()
+    hit = want > 1
+    assert hit.sum() > 500 and (want[hit] <= 1 + base.shape[0]).all()  # the first copy wins
+
+
+def test_next_hit_empty_and_ragged(c1):
+    sc, _, rays = c1
+    for n in (0, 1, 63, 65, 1000):
+        if n == 0:
+            import torch
+
+            rays_d = torch.empty((0, 136), dtype=torch.uint8, device="cuda")
+            hitter = api.HIPBruteForceHitter(api.to_device(sc.ntris()))
+            tr = api.HIPTracer(rays_d)
+            api.next_hit_(tr, hitter, rays_d)
+            continue
+        want, _ = O.next_hit(rays[:n], sc.ntris())
+        assert (_bf_hits(rays[:n], sc.ntris()) == want).all()
+
+
+def test_next_hit_d_count_limits_rays(c1):
+    import torch
+
+    sc, _, rays = c1
+    want, _ = O.next_hit(rays[:1000], sc.ntris())
+    rays_d = api.to_device(rays)
+    hitter = api.HIPBruteForceHitter(api.to_device(sc.ntris()))
+    tr = api.HIPTracer(rays_d)
+    tr.hit_idx.fill_(-7)
+    cnt = torch.tensor([1000], dtype=torch.int64, device="cuda")
+    api.next_hit_(tr, hitter, rays_d, d_count=cnt)
+    got = tr.hit_idx.cpu().numpy()
+    assert (got[:1000] == want).all() and (got[1000:] == -7).all()
+
+
+@pytest.mark.parametrize("depth", [0, 2, 4, 6])
+def test_next_hit_bv_equals_bf(depth):
+    sc = H.small_c2(3)
+    cam = scene.camera_c2(seed=3, height=64, width=64)
+    rays = O.raygen(cam)
+    pts = H.sample_surface_points(sc.ftris, 2000, seed=12)
+    rays = np.concatenate([rays, H.rays_towards(pts, seed=13), H.random_rays(1000, seed=14, n_tris=sc.n_tris)])
+    want, _ = O.next_hit(rays, sc.ntris())
+    tris_d = api.to_device(sc.ftris)
+    hitter = api.HIPBVHitter(tris_d, host_tris=sc.ftris, depth=depth)
+    rays_d = api.to_device(rays)
+    tr = api.HIPTracer(rays_d)
+    api.next_hit_(tr, hitter, rays_d)
+    _sync()
+    got = tr.hit_idx.cpu().numpy()
+    assert (got == want).all()
+
+
+# ---- evolve ---------------------------------------------------------------------------------
+
+def _evolve_gpu(rays_np, hit, ftris, **kw):
+    import torch
+
+    rays = api.to_device(rays_np)
+    api.evolve_(rays, torch.from_numpy(hit).cuda(), api.to_device(ftris), **kw)
+    _sync()
+    return api.from_device(rays, abi.ADRAY)
+
+
+@pytest.mark.parametrize("force_rand", [0.0, 1.0, None])
+@pytest.mark.parametrize("ior", [abi.IOR_REFERENCE_LINEAR, abi.IOR_CAUCHY])
+def test_evolve_bitexact_first_bounces(c1, force_rand, ior):
+    sc, _, rays = c1
+    opt = abi.optics(ior)
+    r = rays
+    for bounce in (1, 2, 3):
+        hit, _ = O.next_hit(r, sc.ntris())
+        fr = math.nan if force_rand is None else force_rand
+        want = O.evolve(r, hit, sc.ftris, force_rand=fr, seed=21, bounce=bounce, first_diffuse=sc.first_diffuse, opt=opt)
+        got = _evolve_gpu(r, hit, sc.ftris, first_diffuse=sc.first_diffuse, force_rand=force_rand, seed=21,
+                          bounce=bounce, optics=opt)
+        assert got.tobytes() == want.tobytes(), f"bounce {bounce}"
+        r = want
+
+
+def test_evolve_quirk_x_ne_y_bitexact():
+    sc = scene.scene_c1()
+    pts = H.sample_surface_points(sc.ftris[: sc.first_diffuse], 3000, seed=31)
+    rays = H.rays_towards(pts, seed=32)
+    rng = np.random.default_rng(33)
+    rays["x"] = rng.uniform(-1, 1, rays.shape[0]).astype(np.float32)
+    rays["y"] = rng.uniform(-1, 1, rays.shape[0]).astype(np.float32)
+    for f in ("dpdx", "dpdy", "dddx", "dddy", "dddl"):
+        rays[f] = (rng.standard_normal((rays.shape[0], 3)) * 1e-2).astype(np.float32)
+    hit, _ = O.next_hit(rays, sc.ntris())
+    for fr in (0.0, 1.0):
+        want = O.evolve(rays, hit, sc.ftris, force_rand=fr, first_diffuse=sc.first_diffuse)
+        got = _evolve_gpu(rays, hit, sc.ftris, first_diffuse=sc.first_diffuse, force_rand=fr)
+        assert got.tobytes() == want.tobytes()
+
+
+def test_evolve_rndm_array_and_inactive_untouched(c1):
+    import torch
+
+    sc, _, rays = c1
+    r = rays.copy()
+    r["status"][::3] = abi.RAY_DIFFUSE
+    hit, _ = O.next_hit(r, sc.ntris())
+    rndm = np.random.default_rng(4).uniform(0, 1, r.shape[0]).astype(np.float32)
+    want = O.evolve(r, hit, sc.ftris, rndm=rndm, first_diffuse=sc.first_diffuse)
+    rays_d = api.to_device(r)
+    api.evolve_(rays_d, torch.from_numpy(hit).cuda(), api.to_device(sc.ftris), first_diffuse=sc.first_diffuse,
+                rndm=torch.from_numpy(rndm).cuda())
+    got = api.from_device(rays_d, abi.ADRAY)
+    assert got.tobytes() == want.tobytes()
+    assert got[::3].tobytes() == r[::3].tobytes()
+
+
+def test_evolve_matches_fp64(c1):
+    """Per-bounce FP32 (GPU) vs FP64 oracle on the same inputs: t via the new position, and all
+    tangents, within 1e-5 relative on rays whose discrete choices agree and are well conditioned."""
+    sc, _, rays = c1
+    hit, _ = O.next_hit(rays, sc.ntris())
+    got = _evolve_gpu(rays, hit, sc.ftris, first_diffuse=sc.first_diffuse, force_rand=1.0)
+    want = O.evolve(rays, hit, sc.ftris, force_rand=1.0, first_diffuse=sc.first_diffuse, precision="f64")
+    same = (got["status"] == want["status"]) & (got["in_medium"] == want["in_medium"]) & (got["status"] == abi.RAY_ACTIVE)
+    assert same.sum() > 1000
+    for f in abi.VEC_FIELDS:
+        a = got[f][same].astype(np.float64)
+        b = want[f][same]
+        err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-3)
+        assert np.quantile(err, 0.99) < 1e-5, (f, np.quantile(err, 0.99))
+
+
+# ---- partition ------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("R", [0, 1, 255, 256, 257, 1000, 70000])
+def test_partition_bitexact(R):
+    rng = np.random.default_rng(R)
+    r = H.random_rays(R, seed=R) if R else np.zeros(0, dtype=abi.ADRAY)
+    if R:
+        r["status"] = rng.choice([0, 1, 2], R, p=[0.3, 0.5, 0.2]).astype(np.uint8)
+    want, n_want, _ = O.partition(r)
+    import torch
+
+    rays_d = api.to_device(r) if R else torch.empty((0, 136), dtype=torch.uint8, device="cuda")
+    swap = torch.empty_like(rays_d)
+    n = api.partition_(rays_d, swap)
+    assert n == n_want
+    assert api.from_device(rays_d, abi.ADRAY).tobytes() == want.tobytes()
+
+
+# ---- light map ------------------------------------------------------------------------------
+
+def _diffuse_rays(sc, cam, depth=3):
+    rays = O.raygen(cam)
+    p = abi.run_params(depth, sc.first_diffuse, force_rand=1.0)
+    r, hit, _ = O.run_evolution(rays, sc.ftris, p)
+    return r, hit
+
+
+@pytest.mark.parametrize("mode,spectrum,n_delta", [
+    ("rgb", None, 1), ("rgb", None, 2), ("mono", None, 3),
+    ("spectral", np.arange(400, 701, 25, dtype=np.float32), 2),
+    ("rgb", np.arange(400, 701, 50, dtype=np.float32), 1),
+])
+def test_light_map_vs_oracle(c1, mode, spectrum, n_delta):
+    import torch
+
+    sc, cam, _ = c1
+    r, hit = _diffuse_rays(sc, cam)
+    lm = abi.lightmap_params(96, 80, n_delta=n_delta, spectrum=spectrum, mode=mode)
+    want_tex, want_rays = O.light_map(r, hit, sc.ftris, lm)
+    rays_d = api.to_device(r)
+    tr = api.HIPTracer(rays_d, N=n_delta)
+    tr.hit_idx.copy_(torch.from_numpy(hit))
+    tex = torch.zeros((lm.channels, lm.height, lm.width), dtype=torch.float32, device="cuda")
+    api.continuum_light_map2_(tracer=tr, tris=api.to_device(sc.ftris), rays=rays_d, tex=tex, lm=lm)
+    _sync()
+    assert api.from_device(rays_d, abi.ADRAY).tobytes() == want_rays.tobytes()  # final_evolution is exact
+    got = tex.cpu().numpy()
+    assert want_tex.sum() > 0
+    assert H.rel_l2(got, want_tex) < 1e-4
+
+
+# ---- the fused bounce loop and the whole frame ---------------------------------------------
+
+@pytest.mark.parametrize("hitter_kind", ["bf", "bv"])
+@pytest.mark.parametrize("depth,force_rand", [(2, 1.0), (4, None), (0, 1.0), (5, 0.0)])
+def test_run_evolution_bitexact(hitter_kind, depth, force_rand):
+    sc = H.small_c2(2)
+    cam = scene.camera_c2(seed=4, height=64, width=48)
+    rays = O.raygen(cam)
+    fr = math.nan if force_rand is None else force_rand
+    p = abi.run_params(depth, sc.first_diffuse, fr, seed=99)
+    want_r, want_h, want_live = O.run_evolution(rays, sc.ftris, p)
+    tris_d = api.to_device(sc.ftris)
+    hitter = api.HIPBVHitter(tris_d, host_tris=sc.ftris, depth=4) if hitter_kind == "bv" else \
+        api.HIPBruteForceHitter(tris_d)
+    rays_d = api.to_device(rays)
+    tr = api.HIPTracer(rays_d)
+    api.run_evolution_(hitter, tr, depth=depth, first_diffuse=sc.first_diffuse, tris=tris_d, rays=rays_d,
+                       force_rand=force_rand, seed=99)
+    _sync()
+    assert int(tr.d_count.item()) == want_live
+    assert (tr.hit_idx.cpu().numpy() == want_h).all()
+    assert api.from_device(rays_d, abi.ADRAY).tobytes() == want_r.tobytes()
+
+
+def test_trace_frame_vs_oracle():
+    sc = H.small_c2(2)
+    cam = scene.camera_c2(seed=6, height=96, width=96)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=5)
+    lm = abi.lightmap_params(128, 128)
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem = bv.bv_partition(sc.ftris, 4)
+    hitter = api.HIPBVHitter(tris_d, bvs, mem)
+    fr = api.Frame(cam, hitter, tris_d, p, lm)
+    fr.trace()
+    _sync()
+    rays = O.raygen(cam)
+    want_r, want_h, _ = O.run_evolution(rays, sc.ftris, p)
+    want_tex, want_r2 = O.light_map(want_r, want_h, sc.ftris, lm)
+    assert (fr.hit_idx.cpu().numpy() == want_h).all()
+    assert api.from_device(fr.rays, abi.ADRAY).tobytes() == want_r2.tobytes()
+    got = fr.tex.cpu().numpy()
+    assert want_tex.sum() > 0 and H.rel_l2(got, want_tex) < 1e-4
+    fr.trace()  # a second frame on the same buffers gives the same map (tex is cleared per frame)
+    _sync()
+    assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
