@@ -121,6 +121,9 @@ typedef struct splat_hitset {
   void* d_recs;          /* device: prepared triangle records, 64 B each                 */
   void* d_boxes;         /* device: conservative member boxes, 32 B each (BV only)       */
   void* d_ranges;        /* device: int32 (first record, count) per BV (BV only)          */
+  void* d_groups;        /* device: boxes of groups of consecutive BVs, 32 B each (BV only) */
+  int32_t group_size;    /* BVs per group; 0 = no coarse level (BV only)                  */
+  int32_t n_groups;      /* ceil(n_bv / group_size)                                        */
   int32_t tune_waves;    /* triangle-split waves per workgroup (0 = autotuned default)   */
   int32_t _pad;
   /* Optional device uint64[4] counters, accumulated by every hitter launch when non-NULL:
@@ -199,6 +202,9 @@ const char* splat_strerror(int32_t code);
 int32_t splat_abi_version(void);
 /* Writes sizeof of (adray, tri, ntri, ftri, bvbox, sphere) into out[0..5]. */
 int32_t splat_layout_sizes(int64_t* out6);
+/* Writes sizeof of (optics, hitset, lightmap_params, pinhole, run_params, frame_request) into
+ * out[0..5], for bindings that mirror these structs. */
+int32_t splat_param_sizes(int64_t* out6);
 /* Device count visible to HIP (0 when none; never fails). */
 int32_t splat_device_count(void);
 
@@ -214,11 +220,14 @@ int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t n_tris_le
 /* Bounding-volume preparation from the reference's BV memberships (DPBVHitter constructor,
  * src/alg_types.jl:81-90, pack_bv_tris src/bv.jl:126-136). `bv_tri_count` (n_bv) and
  * `bv_tris` (Julia column-major n_bv x bv_stride matrix of 1-based indices) are HOST arrays.
- * The boxes are rebuilt from the members and padded so the test is conservative. */
+ * The boxes are rebuilt from the members and padded so the test is conservative. A coarse
+ * level groups `group_size` consecutive BVs under one box (0 = choose ~sqrt(n_bv), negative =
+ * none: the reference's one-level structure). Any memberships that cover the triangles give
+ * the brute-force result. */
 size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members);
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t n_tris_len, int32_t n_bv,
                                 const int64_t* bv_tri_count, const int64_t* bv_tris,
-                                int64_t bv_stride, void* ws, size_t ws_bytes,
+                                int64_t bv_stride, int32_t group_size, void* ws, size_t ws_bytes,
                                 splat_hitset_t* out, void* stream);
 
 /* next_hit!(tracer, hitter, rays, n_tris) (src/hitters.jl:24-30, :34-82, :168-226, :615-648):
@@ -283,6 +292,49 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
                           int32_t n_tris_len, int32_t* hit_idx, const splat_run_params_t* params,
                           const splat_lightmap_params_t* lm, float* tex, void* ws,
                           size_t ws_bytes, void* stream);
+
+/* ---- multi-frame engine: batching and continuous serving --------------------------------
+ * Frames (each with its own resident scene, camera, depth and light map) occupy slots of an
+ * engine. One step advances every admitted frame by one stage -- raygen + bounce 1, bounce k,
+ * ..., final hit + splat -- in ONE fixed sequence of launches shared by all frames: the hitter
+ * is a persistent kernel that pulls 64-ray work items of every frame from a device work queue.
+ * A step has fixed launch shapes, so it is captured once into a HIP graph and replayed. Frames
+ * can be admitted between any two steps (continuous batching); each frame's result is the same
+ * as splat_trace_frame's for the same request. The engine owns one device arena (allocated at
+ * creation, the only allocation) from which each admitted frame takes its ray buffers;
+ * admission fails with SPLAT_ERR_BUDGET when the arena or the slot table is full. */
+typedef struct splat_engine splat_engine_t;
+
+typedef struct splat_frame_request {
+  splat_pinhole_t cam;       /* the frame's rays                                      */
+  splat_hitset_t hs;         /* prepared BV hit set of a resident scene (SPLAT_HITTER_BV) */
+  const splat_ftri_t* tris;  /* device: the scene's FTri array                         */
+  int32_t n_tris;
+  int32_t depth;             /* bounces, <= SPLAT_ENGINE_MAX_DEPTH                      */
+  int32_t first_diffuse;
+  float force_rand;          /* NaN = seeded per-ray random numbers                    */
+  uint64_t seed;
+  splat_optics_t optics;
+  float* tex;                /* device light map (channels x height x width), cleared by the engine */
+  splat_adray_t* rays_out;   /* device, nullable: final rays (else kept in the arena)   */
+  int32_t* hit_out;          /* device, nullable: final hit indices                     */
+  int64_t tag;               /* caller's id, reported back by splat_engine_poll         */
+  splat_lightmap_params_t lm;
+} splat_frame_request_t;
+
+#define SPLAT_ENGINE_MAX_DEPTH 16
+size_t splat_engine_frame_bytes(const splat_pinhole_t* cam, int32_t with_outputs);
+int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine_t** out);
+int32_t splat_engine_destroy(splat_engine_t* e);
+int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* req, void* stream);
+int32_t splat_engine_step(splat_engine_t* e, void* stream);
+/* Reports the tags of frames whose final step has completed (wait != 0: block until every
+ * launched step is done). Their slots and arena space are released. */
+int32_t splat_engine_poll(splat_engine_t* e, int32_t wait, int64_t* tags, int32_t max_tags, int32_t* n_out);
+int32_t splat_engine_in_flight(const splat_engine_t* e);
+int64_t splat_engine_arena_free(const splat_engine_t* e);
+/* on != 0 (default): each step replays one captured HIP graph; 0: plain stream launches. */
+int32_t splat_engine_use_graph(splat_engine_t* e, int32_t on);
 
 #ifdef __cplusplus
 }

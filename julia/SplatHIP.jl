@@ -1,0 +1,118 @@
+# SplatHIP.jl -- the binding a SPLAT maintainer adds to call the MI355X light-map path.
+#
+# New hitter/tracer types plug into the reference's dispatch (src/alg_types.jl) so that
+# run_evolution!/trace! pick the gfx950 kernels, through `ccall` into libsplat_hip.so
+# (include/splat_hip.h). Device arrays are AMDGPU.jl ROCArrays holding the reference's own isbits
+# types (ADRay, FTri, Tuple{Int32,Tri}), passed by pointer with zero copies; the layouts match
+# byte for byte (136/120/52 bytes). Not run in this repository: there is no Julia toolchain here.
+module SplatHIP
+
+using AMDGPU
+
+const LIB = joinpath(@__DIR__, "..", "splat_amd", "libsplat_hip.so")
+
+struct SplatError <: Exception
+    code::Int32
+    where::String
+end
+Base.showerror(io::IO, e::SplatError) =
+    print(io, e.where, ": ", unsafe_string(ccall((:splat_strerror, LIB), Cstring, (Int32,), e.code)))
+check(code, where) = code == 0 || throw(SplatError(code, where))
+
+stream_ptr() = AMDGPU.stream().stream  # the task-local HIP stream handle
+
+# ---- C structs (include/splat_hip.h) -----------------------------------------------------------
+struct Optics
+    ior_model::Int32
+    cauchy_a::Float32
+    cauchy_b::Float32
+    compat::Int32
+end
+Optics(; model = 0, a = 1.5f0, b = 0.004f0, compat = 3) = Optics(model, a, b, compat)  # 0 = the reference's glass()
+
+mutable struct HitSet
+    kind::Int32; n_tris::Int32; n_bv::Int32; max_members::Int32
+    total_members::Int64
+    d_recs::Ptr{Cvoid}; d_boxes::Ptr{Cvoid}; d_ranges::Ptr{Cvoid}; d_groups::Ptr{Cvoid}
+    group_size::Int32; n_groups::Int32; tune_waves::Int32; _pad::Int32
+    d_stats::Ptr{UInt64}
+    HitSet() = new(0, 0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, C_NULL)
+end
+
+struct RunParams
+    depth::Int32; first_diffuse::Int32; force_rand::Float32; _pad::Int32
+    seed::UInt64
+    optics::Optics
+    hit_events::Ptr{Ptr{Cvoid}}; n_hit_events::Int32; _pad2::Int32
+end
+
+# ---- hitters (subtypes of the reference's AbstractHitter) -------------------------------------
+struct HIPBruteForceHitter <: AbstractHitter
+    hs::HitSet
+    ws::ROCArray{UInt8}
+end
+function HIPBruteForceHitter(n_tris::ROCArray{Tuple{Int32,Tri}})
+    N = length(n_tris)
+    ws = ROCArray{UInt8}(undef, ccall((:splat_hitset_bf_bytes, LIB), Csize_t, (Int32,), N))
+    hs = HitSet()
+    check(ccall((:splat_hitset_prepare_bf, LIB), Int32,
+                (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Csize_t, Ref{HitSet}, Ptr{Cvoid}),
+                pointer(n_tris), N, pointer(ws), length(ws), hs, stream_ptr()), "prepare_bf")
+    HIPBruteForceHitter(hs, ws)
+end
+
+struct HIPBVHitter <: AbstractHitter
+    hs::HitSet
+    ws::ROCArray{UInt8}
+end
+# bvs/memberships as returned by bv_partition (src/bv.jl:92-121); packed like pack_bv_tris
+function HIPBVHitter(tris::ROCArray{FTri}, bvs, memberships; group_size = 0)
+    counts, bv_tris = pack_bv_tris(Array, Array(tris), bvs, memberships)  # host copies
+    n_bv = length(bvs)
+    total = sum(counts)
+    ws = ROCArray{UInt8}(undef, ccall((:splat_hitset_bv_bytes, LIB), Csize_t, (Int32, Int64), n_bv, total))
+    hs = HitSet()
+    check(ccall((:splat_hitset_prepare_bv, LIB), Int32,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Int64, Int32, Ptr{Cvoid}, Csize_t, Ref{HitSet}, Ptr{Cvoid}),
+                pointer(tris), length(tris), n_bv, counts, bv_tris, size(bv_tris, 2), group_size,
+                pointer(ws), length(ws), hs, stream_ptr()), "prepare_bv")
+    HIPBVHitter(hs, ws)
+end
+
+# next_hit!(tracer, hitter, rays, n_tris) -- src/hitters.jl:24-30 semantics
+function next_hit!(tracer, hitter::Union{HIPBruteForceHitter,HIPBVHitter}, rays::ROCArray{ADRay}, n_tris)
+    check(ccall((:splat_next_hit, LIB), Int32,
+                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{HitSet}, Ptr{Int32}, Ptr{Cvoid}),
+                pointer(rays), length(rays), C_NULL, hitter.hs, pointer(tracer.hit_idx), stream_ptr()), "next_hit")
+end
+
+# partition!(rays, swap; by = r.status != ACTIVE) -- src/partition.jl:122-132, stable
+function partition!(vals::ROCArray{ADRay}, swap::ROCArray{ADRay}; kwargs...)
+    R = length(vals)
+    ws = ROCArray{UInt8}(undef, ccall((:splat_partition_ws_bytes, LIB), Csize_t, (Int64,), R))
+    d = ROCArray{Int64}(undef, 1)
+    h = Ref{Int64}(0)
+    check(ccall((:splat_partition_active, LIB), Int32,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{Int64}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+                pointer(vals), pointer(swap), R, pointer(d), h, pointer(ws), length(ws), stream_ptr()), "partition")
+    return h[]
+end
+
+# run_evolution!(hitter, ::ExperimentalTracer; ...) -- src/tracer.jl:233-282, device-resident counts
+function run_evolution!(hitter::Union{HIPBruteForceHitter,HIPBVHitter}, tracer::ExperimentalTracer;
+                        depth, first_diffuse, tris, rays, force_rand = nothing, seed = 0, optics = Optics(),
+                        reorder = false, kwargs...)
+    reorder && error("reorder belongs to the backward camera pass")
+    R = length(rays)
+    ws = ROCArray{UInt8}(undef, ccall((:splat_run_ws_bytes, LIB), Csize_t, (Int64,), R))
+    live = ROCArray{Int64}(undef, 1)
+    p = RunParams(depth, first_diffuse, isnothing(force_rand) ? NaN32 : Float32(force_rand), 0, seed, optics,
+                  C_NULL, 0, 0)
+    check(ccall((:splat_run_evolution, LIB), Int32,
+                (Ptr{Cvoid}, Int64, Ref{HitSet}, Ptr{Cvoid}, Int32, Ptr{Int32}, Ref{RunParams}, Ptr{Cvoid}, Csize_t,
+                 Ptr{Int64}, Ptr{Cvoid}),
+                pointer(rays), R, hitter.hs, pointer(tris), length(tris), pointer(tracer.hit_idx), p,
+                pointer(ws), length(ws), pointer(live), stream_ptr()), "run_evolution")
+end
+
+end # module

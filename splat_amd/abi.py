@@ -62,7 +62,8 @@ class HitSet(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("n_tris", C.c_int32), ("n_bv", C.c_int32), ("max_members", C.c_int32),
         ("total_members", C.c_int64), ("d_recs", C.c_void_p), ("d_boxes", C.c_void_p),
-        ("d_ranges", C.c_void_p), ("tune_waves", C.c_int32), ("_pad", C.c_int32), ("d_stats", C.c_void_p),
+        ("d_ranges", C.c_void_p), ("d_groups", C.c_void_p), ("group_size", C.c_int32), ("n_groups", C.c_int32),
+        ("tune_waves", C.c_int32), ("_pad", C.c_int32), ("d_stats", C.c_void_p),
     ]
 
 
@@ -94,7 +95,17 @@ class RunParams(C.Structure):
     ]
 
 
-assert C.sizeof(Pinhole) == 96 and C.sizeof(RunParams) == 56 and C.sizeof(HitSet) == 64
+class FrameRequest(C.Structure):
+    _fields_ = [
+        ("cam", Pinhole), ("hs", HitSet), ("tris", C.c_void_p), ("n_tris", C.c_int32), ("depth", C.c_int32),
+        ("first_diffuse", C.c_int32), ("force_rand", C.c_float), ("seed", C.c_uint64), ("optics", Optics),
+        ("tex", C.c_void_p), ("rays_out", C.c_void_p), ("hit_out", C.c_void_p), ("tag", C.c_int64),
+        ("lm", LightMapParams),
+    ]
+
+
+PARAM_STRUCTS = [Optics, HitSet, LightMapParams, Pinhole, RunParams, FrameRequest]
+assert C.sizeof(Pinhole) == 96 and C.sizeof(RunParams) == 56 and C.sizeof(HitSet) == 80
 
 
 def optics(model: int = IOR_CAUCHY, a: float = 1.5, b: float = 0.004, compat: int = COMPAT_REFERENCE) -> Optics:

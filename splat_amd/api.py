@@ -84,7 +84,8 @@ class HIPBVHitter:
     (src/alg_types.jl:81-90, src/hitters.jl:615-648). Memberships are packed as pack_bv_tris
     does (src/bv.jl:126-136); the result equals the brute-force hitter's."""
 
-    def __init__(self, tris, bvs=None, memberships=None, depth: int = 6, host_tris: np.ndarray | None = None):
+    def __init__(self, tris, bvs=None, memberships=None, depth: int = 6, host_tris: np.ndarray | None = None,
+                 group_size: int = 0):
         L.require_gpu()
         _check_rows(tris, abi.FTRI.itemsize, "tris")
         if memberships is None:
@@ -100,7 +101,7 @@ class HIPBVHitter:
         self.hs = abi.HitSet()
         L.check(lib.splat_hitset_prepare_bv(L.ptr(tris), _nrows(tris), counts.shape[0],
                                             counts.ctypes.data_as(C.c_void_p),
-                                            self.mat.ctypes.data_as(C.c_void_p), stride, L.ptr(self.ws),
+                                            self.mat.ctypes.data_as(C.c_void_p), stride, group_size, L.ptr(self.ws),
                                             self.ws.numel(), L.ref(self.hs), L.stream_handle()), "prepare_bv")
 
 
@@ -231,3 +232,78 @@ class Frame:
                                           L.ptr(self.tris), _nrows(self.tris), L.ptr(self.hit_idx), L.ref(self.params),
                                           L.ref(self.lm), L.ptr(self.tex), L.ptr(self.ws), self.ws.numel(),
                                           L.stream_handle(stream)), "trace_frame")
+
+
+class Engine:
+    """Multi-frame engine (batching and continuous serving) over splat_engine_* of the C ABI:
+    frames of different scenes, sizes and depths share every launch; a step advances each
+    admitted frame one stage and replays one captured HIP graph."""
+
+    def __init__(self, arena_bytes: int, max_slots: int = 256, use_graph: bool = True):
+        L.require_gpu()
+        self._lib = L.lib()
+        h = C.c_void_p()
+        L.check(self._lib.splat_engine_create(int(arena_bytes), int(max_slots), C.byref(h)), "engine_create")
+        self.h = h
+        L.check(self._lib.splat_engine_use_graph(self.h, 1 if use_graph else 0), "engine_use_graph")
+        self._keep = {}  # tag -> objects that must outlive the frame
+
+    def frame_bytes(self, cam: abi.Pinhole, with_outputs: bool = False) -> int:
+        return int(self._lib.splat_engine_frame_bytes(L.ref(cam), 1 if with_outputs else 0))
+
+    def admit(self, *, cam: abi.Pinhole, hitter, tris, params: abi.RunParams, lm: abi.LightMapParams, tex, tag: int,
+              rays_out=None, hit_out=None, stream=None) -> bool:
+        """Queues a frame; False when the arena or the slot table is full (SPLAT_ERR_BUDGET)."""
+        rq = abi.FrameRequest()
+        rq.cam = cam
+        rq.hs = hitter.hs
+        rq.tris = tris.data_ptr()
+        rq.n_tris = _nrows(tris)
+        rq.depth = params.depth
+        rq.first_diffuse = params.first_diffuse
+        rq.force_rand = params.force_rand
+        rq.seed = params.seed
+        rq.optics = params.optics
+        rq.tex = tex.data_ptr()
+        rq.rays_out = None if rays_out is None else rays_out.data_ptr()
+        rq.hit_out = None if hit_out is None else hit_out.data_ptr()
+        rq.tag = int(tag)
+        rq.lm = lm
+        code = self._lib.splat_engine_admit(self.h, L.ref(rq), L.stream_handle(stream))
+        if code == -6:
+            return False
+        L.check(code, "engine_admit")
+        self._keep[int(tag)] = (hitter, tris, tex, rays_out, hit_out)
+        return True
+
+    def step(self, stream=None) -> None:
+        L.check(self._lib.splat_engine_step(self.h, L.stream_handle(stream)), "engine_step")
+
+    def poll(self, wait: bool = False, max_tags: int = 4096) -> list[int]:
+        buf = (C.c_int64 * max_tags)()
+        n = C.c_int32()
+        L.check(self._lib.splat_engine_poll(self.h, 1 if wait else 0, C.cast(buf, C.c_void_p), max_tags, C.byref(n)),
+                "engine_poll")
+        tags = [int(buf[i]) for i in range(n.value)]
+        for t in tags:
+            self._keep.pop(t, None)
+        return tags
+
+    @property
+    def in_flight(self) -> int:
+        return int(self._lib.splat_engine_in_flight(self.h))
+
+    @property
+    def arena_free(self) -> int:
+        return int(self._lib.splat_engine_arena_free(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.splat_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

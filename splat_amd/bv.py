@@ -45,6 +45,37 @@ def bv_partition(ftris: np.ndarray, depth: int = 6):
     return bvs, members
 
 
+def object_partition(ftris: np.ndarray, depth: int = 8):
+    """Object-median build: each triangle (except the sentinel) goes to exactly one leaf by its
+    centroid, split at the median along the centroids' widest axis, 2**depth leaves in recursion
+    order (siblings adjacent). Boxes may overlap but nothing is duplicated, so a large triangle
+    (a ground quad) enlarges only its own leaf instead of every leaf it straddles, which is what
+    makes the reference's straddler-duplicating split (bv_partition) cull poorly. Any covering
+    membership gives the same hit results; this one is the fast build for the workloads here.
+    Returns (bvs, memberships) like bv_partition."""
+    verts = ftris["v"][:, 1:4, :].astype(np.float64)
+    cent = verts.mean(axis=1)
+
+    def rec(idx: np.ndarray, d: int) -> list[np.ndarray]:
+        if d == 0 or idx.size <= 1:
+            return [idx]
+        c = cent[idx]
+        ax = int(np.argmax(c.max(axis=0) - c.min(axis=0)))
+        o = idx[np.argsort(c[:, ax], kind="stable")]
+        h = o.size // 2
+        return rec(o[:h], d - 1) + rec(o[h:], d - 1)
+
+    parts = [p for p in rec(np.arange(1, ftris.shape[0], dtype=np.int64), depth) if p.size > 0]
+    bvs = np.zeros(len(parts), dtype=abi.BVBOX)
+    members = []
+    for k, p in enumerate(parts):
+        v = ftris["v"][p][:, 1:4, :].reshape(-1, 3)
+        bvs[k]["min"] = v.min(axis=0)
+        bvs[k]["max"] = v.max(axis=0)
+        members.append(np.sort(p) + 1)
+    return bvs, members
+
+
 def pack_bv_tris(memberships: list[np.ndarray], n_tris: int, max_overcount_factor: float = 4.0):
     """pack_bv_tris (src/bv.jl:126-136): dense n_bv x stride matrix of 1-based indices, Julia
     column-major (element (k, m) at k + m*n_bv), plus the per-BV counts. The stride grows to the

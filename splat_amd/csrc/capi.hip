@@ -91,7 +91,8 @@ void hit_dispatch(const splat_adray_t* rays, int64_t R, const int64_t* d_count, 
     launch_hit_bf(rays, R, d_count, static_cast<const TriRec*>(hs->d_recs), hs->n_tris, hit_idx, w, hs->d_stats, s);
   } else {
     launch_hit_bv(rays, R, d_count, static_cast<const float4*>(hs->d_boxes), static_cast<const int2*>(hs->d_ranges),
-                  static_cast<const TriRec*>(hs->d_recs), hs->n_bv, hit_idx, hs->d_stats, s);
+                  static_cast<const TriRec*>(hs->d_recs), hs->n_bv, static_cast<const float4*>(hs->d_groups),
+                  hs->d_groups ? hs->group_size : 0, hs->d_groups ? hs->n_groups : 0, hit_idx, hs->d_stats, s);
   }
 }
 
@@ -231,13 +232,13 @@ int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t N, void* 
 
 size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members) {
   if (n_bv < 1 || total_members < 0) return 0;
-  return align_up((size_t)total_members * sizeof(TriRec)) + align_up((size_t)n_bv * 2 * sizeof(float4)) +
+  return align_up((size_t)total_members * sizeof(TriRec)) + 2 * align_up((size_t)n_bv * 2 * sizeof(float4)) +
          align_up((size_t)n_bv * sizeof(int2)) + align_up((size_t)(total_members > 0 ? total_members : 1) * sizeof(int32_t));
 }
 
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_bv, const int64_t* bv_tri_count,
-                                const int64_t* bv_tris, int64_t bv_stride, void* ws, size_t ws_bytes, splat_hitset_t* out,
-                                void* stream) {
+                                const int64_t* bv_tris, int64_t bv_stride, int32_t group_size, void* ws, size_t ws_bytes,
+                                splat_hitset_t* out, void* stream) {
   if (!tris || N < 1 || n_bv < 1 || !bv_tri_count || !bv_tris || bv_stride < 0 || !ws || !out) return SPLAT_ERR_INVALID_ARG;
   // flatten the Julia column-major (n_bv x stride) membership matrix; drop the sentinel index 1
   std::vector<int32_t> members;
@@ -259,9 +260,16 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   }
   const int64_t total = (int64_t)members.size();
   if (ws_bytes < splat_hitset_bv_bytes(n_bv, total)) return SPLAT_ERR_WORKSPACE;
+  if (group_size == 0) {  // ~sqrt(n_bv) BVs per group balances the two levels' slab tests
+    group_size = 1;
+    while ((int64_t)group_size * group_size < n_bv) ++group_size;
+  }
+  if (group_size < 0 || n_bv < 4) group_size = 0;
+  const int32_t n_groups = group_size > 0 ? (n_bv + group_size - 1) / group_size : 0;
   Carver c(ws);
   TriRec* recs = c.take<TriRec>((size_t)total);
   float4* boxes = c.take<float4>((size_t)n_bv * 2);
+  float4* groups = c.take<float4>((size_t)n_bv * 2);
   int2* d_ranges = c.take<int2>((size_t)n_bv);
   int32_t* d_members = c.take<int32_t>((size_t)(total > 0 ? total : 1));
   hipStream_t s = S(stream);
@@ -271,6 +279,7 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
     return SPLAT_ERR_HIP;
   launch_prep_members(tris, d_members, total, recs, s);
   launch_bv_boxes(tris, d_members, d_ranges, n_bv, boxes, s);
+  if (n_groups > 0) launch_group_boxes(boxes, n_bv, group_size, groups, s);
   if (int32_t e = launch_status()) return e;
   // the host vectors must outlive the copies
   if (hipStreamSynchronize(s) != hipSuccess) return SPLAT_ERR_HIP;
@@ -283,6 +292,9 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   out->d_recs = recs;
   out->d_boxes = boxes;
   out->d_ranges = d_ranges;
+  out->d_groups = n_groups > 0 ? groups : nullptr;
+  out->group_size = group_size;
+  out->n_groups = n_groups;
   return SPLAT_OK;
 }
 

@@ -1,0 +1,617 @@
+// engine.hip -- many light-map frames in shared launches: batching and continuous serving.
+//
+// Frames live in slots. A slot at stage s (bounces done) takes part in the next step as:
+//   s == 0      : clear its light map, generate its rays, then bounce 1
+//   0 <= s < d  : hit (live rays) -> evolve -> per-frame scan -> stable scatter   (bounce s+1)
+//   s == d      : realigning hit -> copy of the live rays -> final evolution + splat -> free
+// A step is one fixed sequence of launches for every slot at once: a planning kernel turns the
+// device-side live counts into per-kind work-item prefixes; the hitter is persistent and pulls
+// 64-ray items of every frame from a device work queue (frames of any size and scene cost share
+// the chip); the other stages walk their items grid-stride. Launch shapes never change, so one
+// step is captured into a HIP graph once and replayed. The host tracks stages itself (every
+// admitted frame advances exactly one stage per step), so it never reads device state to learn
+// that a frame has finished: an event recorded after each step tells when.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "hitcore.hpp"
+#include "path.hpp"
+#include "scatter.hpp"
+
+namespace spl {
+
+constexpr int MAX_SLOTS = 1024;
+constexpr int MAXD = SPLAT_ENGINE_MAX_DEPTH;
+enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, K_LMAP, N_KINDS };
+constexpr int HIT_ITEM = 64;     // rays per hitter work item (one wave)
+constexpr int CLEAR_ITEM = 4096;  // light-map floats per clear item
+
+struct SlotDev {
+  splat_adray_t* F;        // final rays (the frame's output order)
+  splat_adray_t* S[2];     // live rays, ping-pong by stage parity
+  int32_t* hit_final;
+  int32_t* hit_scratch;
+  int32_t* tile_counts;
+  int32_t* tile_offsets;
+  float* tex;
+  const splat_ftri_t* tris;
+  BVScene scene;
+  int64_t R;
+  int64_t tex_n;
+  int32_t n_tris, first_diffuse, depth;
+  float force_rand;
+  uint64_t seed;
+  splat_optics_t optics;
+  splat_pinhole_t cam;
+};
+
+struct EngineDev {
+  SlotDev* slots;
+  splat_lightmap_params_t* lm;
+  int32_t* stage;      // -1 = empty
+  int64_t* counters;   // [slot][MAXD + 2]: live rays after bounce k (k = 0: R)
+  int32_t* pre;        // [kind][MAX_SLOTS + 1] exclusive prefixes of work items
+  int32_t* head;       // hitter work-queue head
+  int32_t n_slots;
+};
+
+__device__ __forceinline__ int32_t item_slot(const int32_t* __restrict__ pre, int32_t n, int32_t i) {
+  // largest s with pre[s] <= i (slots with no items share their successor's prefix)
+  int32_t lo = 0, hi = n;  // pre[n] = total > i
+  while (hi - lo > 1) {
+    int32_t mid = (lo + hi) >> 1;
+    if (pre[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// one workgroup of MAX_SLOTS threads: per-slot item counts -> exclusive prefixes
+__global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
+  __shared__ int32_t buf[MAX_SLOTS];
+  const int t = threadIdx.x;
+  const bool in = t < E.n_slots;
+  const int32_t s = in ? E.stage[t] : -1;
+  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0, 0};
+  if (s >= 0) {
+    const SlotDev& D = E.slots[t];
+    int64_t* C = E.counters + (int64_t)t * (MAXD + 2);
+    if (s == 0) C[0] = D.R;
+    const int64_t A = s == 0 ? D.R : C[s];
+    if (s == 0) {
+      cnt[K_RAYGEN] = (int32_t)cdiv(D.R, TILE);
+      cnt[K_CLEAR] = (int32_t)cdiv(D.tex_n, CLEAR_ITEM);
+    }
+    cnt[K_HIT] = (int32_t)cdiv(A, HIT_ITEM);
+    if (s < D.depth) cnt[K_TILE] = (int32_t)cdiv(A, TILE);
+    if (s == D.depth) {
+      cnt[K_FINAL] = (int32_t)cdiv(A, TILE);
+      cnt[K_LMAP] = (int32_t)cdiv(D.R, TILE);
+    }
+  }
+  for (int k = 0; k < N_KINDS; ++k) {
+    buf[t] = cnt[k];
+    __syncthreads();
+    for (int off = 1; off < MAX_SLOTS; off <<= 1) {
+      int32_t v = t >= off ? buf[t - off] : 0;
+      __syncthreads();
+      buf[t] += v;
+      __syncthreads();
+    }
+    int32_t* P = E.pre + k * (MAX_SLOTS + 1);
+    if (t < E.n_slots) P[t] = t == 0 ? 0 : buf[t - 1];
+    if (t == 0) P[E.n_slots] = buf[E.n_slots - 1];
+    __syncthreads();
+  }
+  if (t == 0) *E.head = 0;
+}
+
+__global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_RAYGEN * (MAX_SLOTS + 1);
+  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
+    const int32_t s = item_slot(P, n, i);
+    const SlotDev& D = E.slots[s];
+    const int64_t q = (int64_t)(i - P[s]) * TILE + threadIdx.x;
+    if (q < D.R) raygen_one(D.cam, q, D.S[0] + q);
+  }
+  const int32_t* Q = E.pre + K_CLEAR * (MAX_SLOTS + 1);
+  for (int32_t i = blockIdx.x; i < Q[n]; i += gridDim.x) {
+    const int32_t s = item_slot(Q, n, i);
+    const SlotDev& D = E.slots[s];
+    const int64_t b = (int64_t)(i - Q[s]) * CLEAR_ITEM;
+    for (int64_t k = b + threadIdx.x; k < b + CLEAR_ITEM && k < D.tex_n; k += blockDim.x) D.tex[k] = 0.0f;
+  }
+}
+
+// persistent hitter: every wave pulls 64-ray items from the shared queue until it is empty
+__global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_HIT * (MAX_SLOTS + 1);
+  const int32_t total = P[n];
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int32_t mine = 0;
+    if (lane == 0) mine = atomicAdd(E.head, 1);
+    const int32_t i = __builtin_amdgcn_readfirstlane(mine);
+    if (i >= total) break;  // every wave leaves once the queue is drained
+    const int32_t s = item_slot(P, n, i);
+    const SlotDev& D = E.slots[s];
+    const int32_t st = E.stage[s];
+    const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
+    const int64_t r = (int64_t)(i - P[s]) * HIT_ITEM + lane;
+    const bool valid = r < A;
+    const RayLanes L = load_ray_lanes(D.S[st & 1], r, valid);
+    HitStats hs;
+    const int32_t bi = D.scene.groups ? bv_traverse<false, true>(L, valid, D.scene, hs)
+                                      : bv_traverse<false, false>(L, valid, D.scene, hs);
+    if (valid) (st == D.depth ? D.hit_final : D.hit_scratch)[r] = bi;
+  }
+}
+
+__global__ __launch_bounds__(TILE) void k_b_evolve(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
+  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
+    const int32_t s = item_slot(P, n, i);
+    const SlotDev& D = E.slots[s];
+    const int32_t st = E.stage[s];
+    const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
+    const int32_t tile = i - P[s];
+    const int64_t r = (int64_t)tile * TILE + threadIdx.x;
+    splat_adray_t* rays = D.S[st & 1];
+    bool active_after = false;
+    if (r < A && rays[r].status == SPLAT_RAY_ACTIVE) {
+      RayF ray = load_rayf(rays + r);
+      int32_t h = D.hit_scratch[r];
+      if (h < 1 || h > D.n_tris) h = 1;
+      const FTriF F = load_ftri(D.tris + (h - 1));
+      const float u = bounce_random(D.force_rand, nullptr, r, D.seed, st + 1, ray.dest);
+      RayF o = evolve_one(ray, h, F, u, D.first_diffuse, D.optics);
+      store_rayf(rays + r, o);
+      active_after = o.status == SPLAT_RAY_ACTIVE;
+    }
+    const int c = __syncthreads_count(active_after);
+    if (threadIdx.x == 0) D.tile_counts[tile] = c;
+  }
+}
+
+// one workgroup per slot: exclusive scan of its live tiles' ACTIVE counts
+__global__ __launch_bounds__(1024) void k_b_scan(EngineDev E) {
+  __shared__ int64_t sums[1024];
+  const int32_t s = blockIdx.x;
+  const int32_t st = E.stage[s];
+  if (st < 0) return;
+  const SlotDev& D = E.slots[s];
+  if (st >= D.depth) return;
+  int64_t* C = E.counters + (int64_t)s * (MAXD + 2);
+  const int64_t n_tiles = cdiv(C[st], TILE);
+  const int t = threadIdx.x;
+  const int64_t chunk = (n_tiles + 1023) / 1024;
+  const int64_t b = t * chunk, e = b + chunk < n_tiles ? b + chunk : n_tiles;
+  int64_t sum = 0;
+  for (int64_t i = b; i < e; ++i) sum += D.tile_counts[i];
+  sums[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int64_t v = t >= off ? sums[t - off] : 0;
+    __syncthreads();
+    sums[t] += v;
+    __syncthreads();
+  }
+  int64_t run = sums[t] - sum;
+  for (int64_t i = b; i < e; ++i) {
+    D.tile_offsets[i] = (int32_t)run;
+    run += D.tile_counts[i];
+  }
+  if (t == 1023) C[st + 1] = sums[1023];
+}
+
+__global__ __launch_bounds__(TILE) void k_b_scatter(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
+  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
+    const int32_t s = item_slot(P, n, i);
+    const SlotDev& D = E.slots[s];
+    const int32_t st = E.stage[s];
+    const int64_t* C = E.counters + (int64_t)s * (MAXD + 2);
+    const int64_t A = C[st];
+    const int32_t tile = i - P[s];
+    const int64_t base = (int64_t)tile * TILE;
+    const int nvalid = (int)(A - base < TILE ? A - base : TILE);
+    const int64_t off_a = D.tile_offsets[tile];
+    const int64_t off_r = C[st + 1] + (base - off_a);
+    scatter_tile(D.S[st & 1] + base, nvalid, off_a, off_r, D.S[(st + 1) & 1], D.F, D.hit_scratch + base, D.hit_final);
+  }
+}
+
+// final stage: live rays to their final slots, then final evolution + splat over the frame
+__global__ __launch_bounds__(TILE) void k_b_final(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_FINAL * (MAX_SLOTS + 1);
+  constexpr int W = sizeof(splat_adray_t) / 8;
+  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
+    const int32_t s = item_slot(P, n, i);
+    const SlotDev& D = E.slots[s];
+    const int32_t st = E.stage[s];
+    const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
+    const int64_t base = (int64_t)(i - P[s]) * TILE;
+    const int64_t nw = (A - base < TILE ? A - base : TILE) * W;
+    const uint2* src = reinterpret_cast<const uint2*>(D.S[st & 1] + base);
+    uint2* dst = reinterpret_cast<uint2*>(D.F + base);
+    for (int64_t k = threadIdx.x; k < nw; k += TILE) dst[k] = src[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_b_lmap(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_LMAP * (MAX_SLOTS + 1);
+  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
+    const int32_t s = item_slot(P, n, i);
+    const SlotDev& D = E.slots[s];
+    const int64_t r = (int64_t)(i - P[s]) * TILE + threadIdx.x;
+    if (r >= D.R || D.F[r].status != SPLAT_RAY_DIFFUSE) continue;
+    RayF ray = load_rayf(D.F + r);
+    int32_t h = D.hit_final[r];
+    if (h < 1 || h > D.n_tris) h = 1;
+    final_evolve_and_splat(ray, h, load_ftri(D.tris + (h - 1)), E.lm[s], D.tex, D.F + r);
+  }
+}
+
+__global__ void k_b_finish(EngineDev E) {
+  for (int s = threadIdx.x; s < E.n_slots; s += blockDim.x) {
+    const int32_t st = E.stage[s];
+    if (st < 0) continue;
+    E.stage[s] = st >= E.slots[s].depth ? -1 : st + 1;
+  }
+}
+
+// ---- host side --------------------------------------------------------------------------------
+constexpr size_t ALIGNB = 256;
+inline size_t al(size_t x) { return (x + ALIGNB - 1) & ~(ALIGNB - 1); }
+
+struct Block {
+  int64_t off, size;
+};
+
+// first-fit allocator over the arena, with neighbour merging on free
+struct Arena {
+  int64_t size = 0;
+  std::map<int64_t, int64_t> free_;  // offset -> size
+  void init(int64_t n) { size = n; free_.clear(); free_[0] = n; }
+  bool alloc(int64_t n, int64_t* off) {
+    n = (int64_t)al((size_t)n);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        int64_t rest = it->second - n, o = it->first + n;
+        free_.erase(it);
+        if (rest > 0) free_[o] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(int64_t off, int64_t n) {
+    n = (int64_t)al((size_t)n);
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+    }
+  }
+  int64_t free_bytes() const {
+    int64_t s = 0;
+    for (auto& kv : free_) s += kv.second;
+    return s;
+  }
+};
+
+struct HostSlot {
+  bool used = false;
+  int32_t stage = -1;  // host mirror: stage the slot runs at in the next step
+  int32_t depth = 0;
+  int64_t tag = 0;
+  std::vector<Block> blocks;
+};
+
+struct Pending {
+  hipEvent_t ev;
+  std::vector<int32_t> finishing;  // slots whose last stage ran in this step
+};
+
+size_t frame_bytes(int64_t R, bool with_outputs) {
+  int64_t tiles = (R + TILE - 1) / TILE;
+  size_t b = 2 * al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 4) + 2 * al((size_t)tiles * 4);
+  if (with_outputs) b += al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 4);
+  return b;
+}
+
+}  // namespace spl
+
+using namespace spl;
+
+struct splat_engine {
+  int32_t max_slots = 0;
+  char* arena_base = nullptr;
+  Arena arena;
+  EngineDev dev{};
+  char* dev_mem = nullptr;
+  SlotDev* h_slots = nullptr;  // pinned staging of the slot table
+  splat_lightmap_params_t* h_lm = nullptr;
+  int32_t* h_stage0 = nullptr;
+  std::vector<HostSlot> slots;
+  std::deque<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  hipGraphExec_t graph = nullptr;
+  hipStream_t cap_stream = nullptr;  // the graph is captured here (the null stream cannot capture)
+  int use_graph = 1;
+  int grid = 1024;
+  int hit_grid = 2048;
+};
+
+namespace {
+
+int32_t launch_step_kernels(splat_engine* e, hipStream_t s) {
+  const EngineDev& E = e->dev;
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
+  hipLaunchKernelGGL(k_b_raygen, dim3(e->grid), dim3(256), 0, s, E);
+  hipLaunchKernelGGL(k_b_hit, dim3(e->hit_grid), dim3(256), 0, s, E);
+  hipLaunchKernelGGL(k_b_evolve, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
+  hipLaunchKernelGGL(k_b_scatter, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_final, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_lmap, dim3(e->grid), dim3(256), 0, s, E);
+  hipLaunchKernelGGL(k_b_finish, dim3(1), dim3(1024), 0, s, E);
+  return hipGetLastError() == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
+}
+
+void release_slot(splat_engine* e, int32_t s) {
+  for (const Block& b : e->slots[s].blocks) e->arena.release(b.off, b.size);
+  e->slots[s].blocks.clear();
+  e->slots[s].used = false;
+  e->slots[s].stage = -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t splat_engine_frame_bytes(const splat_pinhole_t* cam, int32_t with_outputs) {
+  if (!cam || cam->height < 1 || cam->width < 1 || cam->samples_per_pixel < 1) return 0;
+  return frame_bytes((int64_t)cam->height * cam->width * cam->samples_per_pixel, with_outputs != 0);
+}
+
+int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine_t** out) {
+  if (!out || arena_bytes <= 0 || max_slots < 1 || max_slots > MAX_SLOTS) return SPLAT_ERR_INVALID_ARG;
+  *out = nullptr;
+  splat_engine* e = new splat_engine();
+  e->max_slots = max_slots;
+  size_t dev_bytes = al(sizeof(SlotDev) * max_slots) + al(sizeof(splat_lightmap_params_t) * max_slots) +
+                     al(sizeof(int32_t) * max_slots) + al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2)) +
+                     al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1)) + al(sizeof(int32_t));
+  if (hipMalloc(&e->arena_base, (size_t)arena_bytes) != hipSuccess) { delete e; return SPLAT_ERR_BUDGET; }
+  if (hipMalloc(&e->dev_mem, dev_bytes) != hipSuccess) { (void)hipFree(e->arena_base); delete e; return SPLAT_ERR_HIP; }
+  if (hipHostMalloc(reinterpret_cast<void**>(&e->h_slots), sizeof(SlotDev) * max_slots) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&e->h_lm), sizeof(splat_lightmap_params_t) * max_slots) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&e->h_stage0), sizeof(int32_t) * max_slots) != hipSuccess) {
+    splat_engine_destroy(e);
+    return SPLAT_ERR_HIP;
+  }
+  e->arena.init(arena_bytes);
+  char* p = e->dev_mem;
+  e->dev.slots = reinterpret_cast<SlotDev*>(p); p += al(sizeof(SlotDev) * max_slots);
+  e->dev.lm = reinterpret_cast<splat_lightmap_params_t*>(p); p += al(sizeof(splat_lightmap_params_t) * max_slots);
+  e->dev.stage = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * max_slots);
+  e->dev.counters = reinterpret_cast<int64_t*>(p); p += al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2));
+  e->dev.pre = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1));
+  e->dev.head = reinterpret_cast<int32_t*>(p);
+  e->dev.n_slots = max_slots;
+  for (int i = 0; i < max_slots; ++i) e->h_stage0[i] = -1;
+  if (hipMemcpy(e->dev.stage, e->h_stage0, sizeof(int32_t) * max_slots, hipMemcpyHostToDevice) != hipSuccess) {
+    splat_engine_destroy(e);
+    return SPLAT_ERR_HIP;
+  }
+  e->h_stage0[0] = 0;
+  e->slots.resize(max_slots);
+  int cus = 256;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  e->grid = cus * 4;
+  e->hit_grid = cus * 8;
+  *out = e;
+  return SPLAT_OK;
+}
+
+int32_t splat_engine_destroy(splat_engine_t* e) {
+  if (!e) return SPLAT_ERR_INVALID_ARG;
+  (void)hipDeviceSynchronize();
+  for (auto& p : e->pending) (void)hipEventDestroy(p.ev);
+  for (auto ev : e->free_events) (void)hipEventDestroy(ev);
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
+  if (e->h_slots) (void)hipHostFree(e->h_slots);
+  if (e->h_lm) (void)hipHostFree(e->h_lm);
+  if (e->h_stage0) (void)hipHostFree(e->h_stage0);
+  if (e->dev_mem) (void)hipFree(e->dev_mem);
+  if (e->arena_base) (void)hipFree(e->arena_base);
+  delete e;
+  return SPLAT_OK;
+}
+
+int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, void* stream) {
+  if (!e || !rq || !rq->tex || !rq->tris || rq->n_tris < 1) return SPLAT_ERR_INVALID_ARG;
+  const splat_pinhole_t& cam = rq->cam;
+  if (cam.height < 1 || cam.width < 1 || cam.samples_per_pixel < 1 || !(cam.fov_half_sin > 0.f)) return SPLAT_ERR_INVALID_ARG;
+  if (rq->hs.kind != SPLAT_HITTER_BV || !rq->hs.d_recs || !rq->hs.d_boxes || !rq->hs.d_ranges) return SPLAT_ERR_INVALID_ARG;
+  if (rq->depth < 0 || rq->depth > MAXD) return SPLAT_ERR_UNSUPPORTED;
+  const splat_lightmap_params_t& lm = rq->lm;
+  if (lm.width < 1 || lm.height < 1 || lm.channels < 1 || lm.n_delta < 1 || lm.n_delta > SPLAT_MAX_DELTA ||
+      lm.n_lambda < 0 || lm.n_lambda > SPLAT_MAX_LAMBDA || lm.weight_mode < 0 || lm.weight_mode > 2)
+    return SPLAT_ERR_INVALID_ARG;
+  if (rq->optics.ior_model != SPLAT_IOR_REFERENCE_LINEAR && rq->optics.ior_model != SPLAT_IOR_CAUCHY) return SPLAT_ERR_INVALID_ARG;
+  const int64_t R = (int64_t)cam.height * cam.width * cam.samples_per_pixel;
+  if (R > (int64_t)INT32_MAX / 2) return SPLAT_ERR_UNSUPPORTED;
+  int32_t slot = -1;
+  for (int32_t i = 0; i < e->max_slots; ++i)
+    if (!e->slots[i].used) { slot = i; break; }
+  if (slot < 0) return SPLAT_ERR_BUDGET;
+  HostSlot& hsl = e->slots[slot];
+  const int64_t tiles = (R + TILE - 1) / TILE;
+  std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 4, tiles * 4, tiles * 4};
+  if (!rq->rays_out) sizes.push_back(R * (int64_t)sizeof(splat_adray_t));
+  if (!rq->hit_out) sizes.push_back(R * 4);
+  std::vector<Block> got;
+  for (int64_t n : sizes) {
+    int64_t off;
+    if (!e->arena.alloc(n, &off)) {
+      for (const Block& b : got) e->arena.release(b.off, b.size);
+      return SPLAT_ERR_BUDGET;
+    }
+    got.push_back(Block{off, n});
+  }
+  auto at = [&](size_t k) { return e->arena_base + got[k].off; };
+  SlotDev& D = e->h_slots[slot];
+  std::memset(&D, 0, sizeof(D));
+  D.S[0] = reinterpret_cast<splat_adray_t*>(at(0));
+  D.S[1] = reinterpret_cast<splat_adray_t*>(at(1));
+  D.hit_scratch = reinterpret_cast<int32_t*>(at(2));
+  D.tile_counts = reinterpret_cast<int32_t*>(at(3));
+  D.tile_offsets = reinterpret_cast<int32_t*>(at(4));
+  size_t k = 5;
+  D.F = rq->rays_out ? rq->rays_out : reinterpret_cast<splat_adray_t*>(at(k++));
+  D.hit_final = rq->hit_out ? rq->hit_out : reinterpret_cast<int32_t*>(at(k++));
+  D.tex = rq->tex;
+  D.tris = rq->tris;
+  D.scene = BVScene{static_cast<const float4*>(rq->hs.d_boxes), static_cast<const int2*>(rq->hs.d_ranges),
+                    static_cast<const TriRec*>(rq->hs.d_recs),
+                    rq->hs.d_groups && rq->hs.group_size > 0 ? static_cast<const float4*>(rq->hs.d_groups) : nullptr,
+                    rq->hs.n_bv, rq->hs.group_size, rq->hs.n_groups};
+  D.R = R;
+  D.tex_n = (int64_t)lm.width * lm.height * lm.channels;
+  D.n_tris = rq->n_tris;
+  D.first_diffuse = rq->first_diffuse;
+  D.depth = rq->depth;
+  D.force_rand = rq->force_rand;
+  D.seed = rq->seed;
+  D.optics = rq->optics;
+  D.cam = cam;
+  e->h_lm[slot] = lm;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(e->dev.slots + slot, &D, sizeof(SlotDev), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(e->dev.lm + slot, e->h_lm + slot, sizeof(splat_lightmap_params_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(e->dev.stage + slot, e->h_stage0, sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess) {
+    for (const Block& b : got) e->arena.release(b.off, b.size);
+    return SPLAT_ERR_HIP;
+  }
+  hsl.used = true;
+  hsl.stage = 0;
+  hsl.depth = rq->depth;
+  hsl.tag = rq->tag;
+  hsl.blocks = got;
+  return SPLAT_OK;
+}
+
+int32_t splat_engine_step(splat_engine_t* e, void* stream) {
+  if (!e) return SPLAT_ERR_INVALID_ARG;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  bool any = false;
+  for (auto& h : e->slots) any |= h.used && h.stage >= 0;
+  if (!any) return SPLAT_OK;
+  if (e->use_graph) {
+    if (!e->graph) {
+      hipGraph_t g;
+      if (!e->cap_stream && hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess) return SPLAT_ERR_HIP;
+      if (hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPLAT_ERR_HIP;
+      int32_t st = launch_step_kernels(e, e->cap_stream);
+      if (hipStreamEndCapture(e->cap_stream, &g) != hipSuccess || st != SPLAT_OK) return SPLAT_ERR_HIP;
+      hipError_t ie = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return SPLAT_ERR_HIP;
+    }
+    if (hipGraphLaunch(e->graph, s) != hipSuccess) return SPLAT_ERR_HIP;
+  } else if (int32_t st = launch_step_kernels(e, s)) {
+    return st;
+  }
+  Pending p;
+  if (e->free_events.empty()) {
+    if (hipEventCreateWithFlags(&p.ev, hipEventDisableTiming) != hipSuccess) return SPLAT_ERR_HIP;
+  } else {
+    p.ev = e->free_events.back();
+    e->free_events.pop_back();
+  }
+  for (int32_t i = 0; i < e->max_slots; ++i) {
+    HostSlot& h = e->slots[i];
+    if (!h.used || h.stage < 0) continue;
+    if (h.stage >= h.depth) {
+      p.finishing.push_back(i);
+      h.stage = -2;  // finished on the device once this step's event completes
+    } else {
+      ++h.stage;
+    }
+  }
+  if (hipEventRecord(p.ev, s) != hipSuccess) return SPLAT_ERR_HIP;
+  e->pending.push_back(std::move(p));
+  return SPLAT_OK;
+}
+
+int32_t splat_engine_poll(splat_engine_t* e, int32_t wait, int64_t* tags, int32_t max_tags, int32_t* n_out) {
+  if (!e || !n_out || (max_tags > 0 && !tags)) return SPLAT_ERR_INVALID_ARG;
+  *n_out = 0;
+  while (!e->pending.empty()) {
+    Pending& p = e->pending.front();
+    if ((int32_t)p.finishing.size() > max_tags - *n_out) break;  // caller's buffer is full
+    if (wait) {
+      if (hipEventSynchronize(p.ev) != hipSuccess) return SPLAT_ERR_HIP;
+    } else {
+      hipError_t q = hipEventQuery(p.ev);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) return SPLAT_ERR_HIP;
+    }
+    for (int32_t sl : p.finishing) {
+      tags[(*n_out)++] = e->slots[sl].tag;
+      release_slot(e, sl);
+    }
+    e->free_events.push_back(p.ev);
+    e->pending.pop_front();
+  }
+  return SPLAT_OK;
+}
+
+int32_t splat_engine_in_flight(const splat_engine_t* e) {
+  if (!e) return 0;
+  int32_t n = 0;
+  for (auto& h : e->slots) n += h.used ? 1 : 0;
+  return n;
+}
+
+int64_t splat_engine_arena_free(const splat_engine_t* e) { return e ? e->arena.free_bytes() : 0; }
+
+int32_t splat_param_sizes(int64_t* out6) {
+  if (!out6) return SPLAT_ERR_INVALID_ARG;
+  out6[0] = sizeof(splat_optics_t);
+  out6[1] = sizeof(splat_hitset_t);
+  out6[2] = sizeof(splat_lightmap_params_t);
+  out6[3] = sizeof(splat_pinhole_t);
+  out6[4] = sizeof(splat_run_params_t);
+  out6[5] = sizeof(splat_frame_request_t);
+  return SPLAT_OK;
+}
+
+int32_t splat_engine_use_graph(splat_engine_t* e, int32_t on) {
+  if (!e) return SPLAT_ERR_INVALID_ARG;
+  e->use_graph = on != 0;
+  return SPLAT_OK;
+}
+
+}  // extern "C"

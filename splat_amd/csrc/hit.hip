@@ -11,7 +11,7 @@
 //    global atomics, ties to the lowest index;
 //  * the BV hitter walks the conservative member boxes per wave; a box whose slab test fails for
 //    all 64 lanes (one ballot) is skipped, otherwise its member records are scanned.
-#include "internal.hpp"
+#include "hitcore.hpp"
 
 namespace spl {
 
@@ -83,28 +83,6 @@ __global__ __launch_bounds__(256) void k_bv_boxes(const splat_ftri_t* __restrict
   }
 }
 
-// ---- ray loading ----------------------------------------------------------------------------
-struct RayLanes {
-  float ox, oy, oz, dx, dy, dz;
-  int64_t ig;
-};
-__device__ __forceinline__ RayLanes load_ray_lanes(const splat_adray_t* __restrict__ rays, int64_t r, bool valid) {
-  RayLanes L{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
-  if (valid) {
-    const splat_adray_t* p = rays + r;
-    L.ox = p->pos[0]; L.oy = p->pos[1]; L.oz = p->pos[2];
-    L.dx = p->dir[0]; L.dy = p->dir[1]; L.dz = p->dir[2];
-    L.ig = p->ignore_tri;
-  }
-  return L;
-}
-
-__device__ __forceinline__ int64_t live_count(int64_t R, const int64_t* __restrict__ d_count) {
-  if (!d_count) return R;
-  int64_t c = *d_count;
-  return c < R ? c : R;
-}
-
 // ---- brute force ------------------------------------------------------------------------------
 // Workgroup = W waves sharing 64 rays; wave w scans record slots 1+w, 1+w+W, ... (slot 0 is the
 // degenerate sentinel). Within a wave indices increase, so a strict t < best keeps the lowest
@@ -160,65 +138,46 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
   }
 }
 
-// ---- one-level bounding volumes ------------------------------------------------------------
-template <bool STATS>
+template <bool STATS, bool GROUPS>
 __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
                                                 const int64_t* __restrict__ d_count,
                                                 const float4* __restrict__ boxes, const int2* __restrict__ ranges,
                                                 const TriRec* __restrict__ recs, int32_t n_bv,
+                                                const float4* __restrict__ groups, int32_t gsize, int32_t n_groups,
                                                 int32_t* __restrict__ hit_idx, unsigned long long* stats) {
   const int64_t Rl = live_count(R, d_count);
   const int64_t wave_base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
   if (wave_base >= Rl) return;  // uniform over the wave
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool valid = r < Rl;
-  RayLanes L = load_ray_lanes(rays, r, valid);
-  // slab-test reciprocals; |d| < 1e-30 is clamped so 0 * inf never appears
-  auto safe_inv = [](float d) {
-    float a = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
-    return 1.0f / a;
-  };
-  const float ix = safe_inv(L.dx), iy = safe_inv(L.dy), iz = safe_inv(L.dz);
-  float best = INFINITY;
-  int32_t bi = 1;
-  unsigned long long n_enter = 0, n_alg = 0, n_exec = 0;
-  for (int32_t b = 0; b < n_bv; ++b) {
-    const float4 lo = boxes[2 * b], hi = boxes[2 * b + 1];
-    float t1 = (lo.x - L.ox) * ix, t2 = (hi.x - L.ox) * ix;
-    float tmin = fmaxf(0.0f, fminf(t1, t2)), tmax = fmaxf(t1, t2);
-    t1 = (lo.y - L.oy) * iy; t2 = (hi.y - L.oy) * iy;
-    tmin = fmaxf(tmin, fminf(t1, t2)); tmax = fminf(tmax, fmaxf(t1, t2));
-    t1 = (lo.z - L.oz) * iz; t2 = (hi.z - L.oz) * iz;
-    tmin = fmaxf(tmin, fminf(t1, t2)); tmax = fminf(tmax, fmaxf(t1, t2));
-    const bool enter = valid && (tmin <= tmax) && (tmin <= best);
-    if (__builtin_amdgcn_ballot_w64(enter) == 0) continue;
-    const int2 rg = ranges[b];
-    if constexpr (STATS) {
-      n_enter += enter ? 1ull : 0ull;
-      n_alg += enter ? (unsigned long long)rg.y : 0ull;
-      n_exec += (unsigned long long)rg.y;
-    }
-#pragma unroll 4
-    for (int32_t m = 0; m < rg.y; ++m) {
-      const TriRec& T = recs[rg.x + m];
-      float t;
-      bool h = hit_rec(T.q0, T.q1, T.q2, T.q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
-      int32_t idx = __float_as_int(T.q1.w);
-      if (h && (t < best || (t == best && idx < bi))) {
-        best = t;
-        bi = idx;
-      }
-    }
-  }
+  const RayLanes L = load_ray_lanes(rays, r, valid);
+  HitStats st;
+  const int32_t bi = bv_traverse<STATS, GROUPS>(L, valid, BVScene{boxes, ranges, recs, groups, n_bv, gsize, n_groups}, st);
   if (valid) hit_idx[r] = bi;
   if constexpr (STATS) {
     if (valid) {
-      atomicAdd(stats + 0, (unsigned long long)n_bv);
-      atomicAdd(stats + 1, n_enter);
-      atomicAdd(stats + 2, n_alg);
+      atomicAdd(stats + 0, st.n_box);
+      atomicAdd(stats + 1, st.n_enter);
+      atomicAdd(stats + 2, st.n_alg);
     }
-    if ((threadIdx.x & 63) == 0) atomicAdd(stats + 3, 64ull * n_exec);
+    if ((threadIdx.x & 63) == 0) atomicAdd(stats + 3, 64ull * st.n_exec);
   }
+}
+
+// group boxes: union of each group's (already padded) leaf boxes
+__global__ void k_group_boxes(const float4* __restrict__ boxes, int32_t n_bv, int32_t gsize, float4* __restrict__ groups) {
+  const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t n_groups = (n_bv + gsize - 1) / gsize;
+  if (g >= n_groups) return;
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  const int32_t b1 = (g + 1) * gsize < n_bv ? (g + 1) * gsize : n_bv;
+  for (int32_t b = g * gsize; b < b1; ++b) {
+    const float4 l = boxes[2 * b], h = boxes[2 * b + 1];
+    lo.x = fminf(lo.x, l.x); lo.y = fminf(lo.y, l.y); lo.z = fminf(lo.z, l.z);
+    hi.x = fmaxf(hi.x, h.x); hi.y = fmaxf(hi.y, h.y); hi.z = fmaxf(hi.z, h.z);
+  }
+  groups[2 * g] = lo;
+  groups[2 * g + 1] = hi;
 }
 
 // ---- launchers -------------------------------------------------------------------------------
@@ -267,15 +226,26 @@ void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
   else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, waves, st, s);
 }
 
+void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t gsize, float4* groups, hipStream_t s) {
+  const int32_t n_groups = (n_bv + gsize - 1) / gsize;
+  hipLaunchKernelGGL(k_group_boxes, dim3((n_groups + 63) / 64), dim3(64), 0, s, boxes, n_bv, gsize, groups);
+}
+
 void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const float4* boxes, const int2* ranges,
-                   const TriRec* recs, int32_t n_bv, int32_t* hit_idx, uint64_t* stats, hipStream_t s) {
+                   const TriRec* recs, int32_t n_bv, const float4* groups, int32_t gsize, int32_t n_groups,
+                   int32_t* hit_idx, uint64_t* stats, hipStream_t s) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
-  dim3 grid((unsigned)((R + 255) / 256));
-  if (st)
-    hipLaunchKernelGGL(k_hit_bv<true>, grid, dim3(256), 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, hit_idx, st);
+  dim3 grid((unsigned)((R + 255) / 256)), block(256);
+  const bool g = groups != nullptr && gsize > 0 && n_groups > 0;
+  if (st && g)
+    hipLaunchKernelGGL((k_hit_bv<true, true>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
+  else if (st)
+    hipLaunchKernelGGL((k_hit_bv<true, false>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
+  else if (g)
+    hipLaunchKernelGGL((k_hit_bv<false, true>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
   else
-    hipLaunchKernelGGL(k_hit_bv<false>, grid, dim3(256), 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, hit_idx, st);
+    hipLaunchKernelGGL((k_hit_bv<false, false>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
 }
 
 }  // namespace spl

@@ -16,8 +16,10 @@ void launch_bv_boxes(const splat_ftri_t* tris, const int32_t* members, const int
 int choose_bf_waves(int64_t R, int32_t N);
 void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
                    int32_t* hit_idx, int waves, uint64_t* stats, hipStream_t s);
+void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, float4* groups, hipStream_t s);
 void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const float4* boxes, const int2* ranges,
-                   const TriRec* recs, int32_t n_bv, int32_t* hit_idx, uint64_t* stats, hipStream_t s);
+                   const TriRec* recs, int32_t n_bv, const float4* groups, int32_t group_size, int32_t n_groups,
+                   int32_t* hit_idx, uint64_t* stats, hipStream_t s);
 
 // trace.hip
 void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,

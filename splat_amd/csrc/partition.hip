@@ -6,11 +6,10 @@
 // exclusive scan over tiles, then a scatter whose in-tile ranks come from wave ballots
 // (v_mbcnt) and a 4-entry LDS prefix. The scatter stages the tile's 136-byte records through LDS
 // so that both the read of the tile and the writes of its two output runs are coalesced.
-#include "internal.hpp"
+#include "scatter.hpp"
 
 namespace spl {
 
-constexpr int WORDS = sizeof(splat_adray_t) / 8;  // 17 eight-byte words per record
 
 __device__ __forceinline__ int64_t live_count_p(int64_t R, const int64_t* __restrict__ d_count) {
   if (!d_count) return R;
@@ -61,54 +60,14 @@ __global__ __launch_bounds__(TILE) void k_scatter(const splat_adray_t* __restric
                                                   splat_adray_t* __restrict__ out_rest,
                                                   const int32_t* __restrict__ carry_in,
                                                   int32_t* __restrict__ carry_rest) {
-  __shared__ uint2 stage[TILE * WORDS];  // 34,816 bytes
-  __shared__ int16_t order[TILE];
-  __shared__ int32_t wave_a[TILE / 64], wave_r[TILE / 64];
   const int64_t Rl = live_count_p(R, d_count);
   const int64_t tile_base = (int64_t)blockIdx.x * TILE;
   if (tile_base >= Rl) return;  // uniform over the workgroup
   const int nvalid = (int)(Rl - tile_base < TILE ? Rl - tile_base : TILE);
-
-  // coalesced read of the tile
-  const uint2* src = reinterpret_cast<const uint2*>(in + tile_base);
-  for (int e = threadIdx.x; e < nvalid * WORDS; e += TILE) stage[e] = src[e];
-  __syncthreads();
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const bool valid = tid < nvalid;
-  const uint8_t status = valid ? reinterpret_cast<const splat_adray_t*>(stage)[tid].status : 0;
-  const bool act = valid && status == SPLAT_RAY_ACTIVE;
-  const bool rest = valid && !act;
-  const uint64_t ma = __builtin_amdgcn_ballot_w64(act), mr = __builtin_amdgcn_ballot_w64(rest);
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const int ra = __builtin_popcountll(ma & below), rr = __builtin_popcountll(mr & below);
-  if (lane == 0) {
-    wave_a[wave] = __builtin_popcountll(ma);
-    wave_r[wave] = __builtin_popcountll(mr);
-  }
-  __syncthreads();
-  int pa = 0, pr = 0, na = 0;
-  for (int w = 0; w < TILE / 64; ++w) {
-    if (w < wave) { pa += wave_a[w]; pr += wave_r[w]; }
-    na += wave_a[w];
-  }
-  if (act) order[pa + ra] = (int16_t)tid;
-  if (rest) order[na + pr + rr] = (int16_t)tid;
-
   const int64_t off_a = tile_offsets[blockIdx.x];
   const int64_t off_r = *d_total + (tile_base - off_a);  // non-ACTIVE rays before this tile
-  if (rest && carry_rest) carry_rest[off_r + pr + rr] = carry_in[tile_base + tid];
-  __syncthreads();
-
-  // coalesced writes of the two output runs
-  uint2* dst_a = reinterpret_cast<uint2*>(out_active + off_a);
-  uint2* dst_r = reinterpret_cast<uint2*>(out_rest + off_r);
-  for (int e = threadIdx.x; e < nvalid * WORDS; e += TILE) {
-    const int slot = e / WORDS, w = e - slot * WORDS;
-    const uint2 v = stage[order[slot] * WORDS + w];
-    if (slot < na) dst_a[e] = v;
-    else dst_r[e - na * WORDS] = v;
-  }
+  scatter_tile(in + tile_base, nvalid, off_a, off_r, out_active, out_rest, carry_in ? carry_in + tile_base : nullptr,
+               carry_rest);
 }
 
 void launch_count_active(const splat_adray_t* rays, int64_t R, const int64_t* d_count, int32_t* tile_counts,

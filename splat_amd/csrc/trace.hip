@@ -1,124 +1,7 @@
 // trace.hip -- dual-number evolve, final evolution + light-map splat, pinhole ray source.
-#include "internal.hpp"
+#include "path.hpp"
 
 namespace spl {
-
-struct RayF {
-  V3<float> pos, dp[3], dir, dd[3];
-  bool in_medium;
-  int64_t ignore_tri, dest;
-  float lambda, x, y;
-  uint8_t status;
-};
-
-__device__ __forceinline__ RayF load_rayf(const splat_adray_t* p) {
-  RayF r;
-  r.pos = ld3(p->pos); r.dp[0] = ld3(p->dpdl); r.dp[1] = ld3(p->dpdx); r.dp[2] = ld3(p->dpdy);
-  r.dir = ld3(p->dir); r.dd[0] = ld3(p->dddl); r.dd[1] = ld3(p->dddx); r.dd[2] = ld3(p->dddy);
-  r.in_medium = p->in_medium != 0; r.ignore_tri = p->ignore_tri; r.dest = p->dest;
-  r.lambda = p->lambda; r.x = p->x; r.y = p->y; r.status = p->status;
-  return r;
-}
-// writes every field a bounce can change (the rest -- dest, lambda, x, y -- is left in place)
-__device__ __forceinline__ void store_rayf(splat_adray_t* p, const RayF& r) {
-  st3(p->pos, r.pos); st3(p->dpdl, r.dp[0]); st3(p->dpdx, r.dp[1]); st3(p->dpdy, r.dp[2]);
-  st3(p->dir, r.dir); st3(p->dddl, r.dd[0]); st3(p->dddx, r.dd[1]); st3(p->dddy, r.dd[2]);
-  p->in_medium = r.in_medium ? 1 : 0; p->ignore_tri = r.ignore_tri; p->status = r.status;
-}
-
-__device__ __forceinline__ int64_t live_count_t(int64_t R, const int64_t* __restrict__ d_count) {
-  if (!d_count) return R;
-  int64_t c = *d_count;
-  return c < R ? c : R;
-}
-
-__device__ __forceinline__ V3<D3<float>> pos_dual(const RayF& r) {
-  return V3<D3<float>>{{r.pos.x, {r.dp[0].x, r.dp[1].x, r.dp[2].x}},
-                       {r.pos.y, {r.dp[0].y, r.dp[1].y, r.dp[2].y}},
-                       {r.pos.z, {r.dp[0].z, r.dp[1].z, r.dp[2].z}}};
-}
-__device__ __forceinline__ V3<D3<float>> dir_dual(const RayF& r) {
-  return V3<D3<float>>{{r.dir.x, {r.dd[0].x, r.dd[1].x, r.dd[2].x}},
-                       {r.dir.y, {r.dd[0].y, r.dd[1].y, r.dd[2].y}},
-                       {r.dir.z, {r.dd[0].z, r.dd[1].z, r.dd[2].z}}};
-}
-// next_p at the expansion point (src/tracer.jl:109-116): p + d t, as duals
-__device__ __forceinline__ V3<D3<float>> next_p_dual(const V3<D3<float>>& P0, const V3<D3<float>>& D0, const D3<float>& t) {
-  return V3<D3<float>>{FMA(D0.x, t, P0.x), FMA(D0.y, t, P0.y), FMA(D0.z, t, P0.z)};
-}
-__device__ __forceinline__ void split(const V3<D3<float>>& a, V3<float>* v, V3<float>* d3) {
-  *v = vval(a);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) d3[k] = V3<float>{a.x.d[k], a.y.d[k], a.z.d[k]};
-}
-
-// get_hit((i, T), ::ADRay) (src/tracer.jl:61-81): hit test on the value, t as a dual
-__device__ __forceinline__ bool get_hit_ad(const RayF& r, const FTriF& F, int32_t i, D3<float>* tD) {
-  TriRec P = tri_prep(F.v[0], F.v[1], F.v[2], F.v[3], i);
-  float tv;
-  if (!hit_rec(P.q0, P.q1, P.q2, P.q3, r.pos.x, r.pos.y, r.pos.z, r.dir.x, r.dir.y, r.dir.z, r.ignore_tri, tv)) {
-    *tD = D3<float>{INFINITY, {INFINITY, INFINITY, INFINITY}};
-    return false;
-  }
-  V3<float> n{P.q0.x, P.q0.y, P.q0.z};
-  D3<float> no = dot3(n, pos_dual(r));
-  D3<float> nd = dot3(n, dir_dual(r));
-  D3<float> num = P.q0.w - no;
-  *tD = num / nd;
-  return true;
-}
-
-// evolve_ray + handle_optics (src/tracer.jl:181-201, :118-179)
-__device__ __forceinline__ RayF evolve_one(const RayF& r, int32_t i, const FTriF& F, float rndm, int32_t first_diffuse,
-                                           const splat_optics_t& opt) {
-  D3<float> tD;
-  bool hit = get_hit_ad(r, F, i, &tD);
-  int32_t i2 = hit ? i : 1;
-  RayF out = r;
-  if (i2 >= first_diffuse) { out.status = SPLAT_RAY_DIFFUSE; return out; }
-  if (!hit) { out.status = SPLAT_RAY_INFINITY; return out; }
-
-  D3<float> g = glass_ior(opt, r.lambda), a{1.0f, {0.0f, 0.0f, 0.0f}};
-  D3<float> n1 = r.in_medium ? g : a, n2 = r.in_medium ? a : g;
-
-  V3<D3<float>> P0 = pos_dual(r), D0 = dir_dual(r);
-  V3<D3<float>> P = next_p_dual(P0, D0, tD);
-  V3<D3<float>> Nd = optical_normal(F, P);
-  V3<float> Nv = vval(Nd);
-  bool refracts = can_refract(r.dir, Nv, n1.v, n2.v) && (rndm > reflectance(r.dir, Nv, n1.v, n2.v));
-
-  V3<D3<float>> Dout;
-  if (refracts) {
-    V3<D3<float>> Nr = Nd;
-    if (opt.compat & SPLAT_COMPAT_REFRACT_NORMAL_LAMBDA) { Nr.x.d[0] = 0.0f; Nr.y.d[0] = 0.0f; Nr.z.d[0] = 0.0f; }
-    Dout = refract(D0, Nr, n1, n2);
-  } else {
-    Dout = reflect(D0, Nd);
-  }
-  split(P, &out.pos, out.dp);
-  split(Dout, &out.dir, out.dd);
-
-  if ((opt.compat & SPLAT_COMPAT_DY_AT_X) && r.x != r.y) {
-    // d dir / d y evaluated at y = r.x, as the reference does (src/tracer.jl:138-141, :166-169)
-    float delta = r.x - r.y;
-    float z = 0.0f;
-    V3<D3<float>> Py0{{FMA(r.dp[2].x, delta, r.pos.x), {z, z, r.dp[2].x}},
-                      {FMA(r.dp[2].y, delta, r.pos.y), {z, z, r.dp[2].y}},
-                      {FMA(r.dp[2].z, delta, r.pos.z), {z, z, r.dp[2].z}}};
-    V3<D3<float>> Dy0{{FMA(r.dd[2].x, delta, r.dir.x), {z, z, r.dd[2].x}},
-                      {FMA(r.dd[2].y, delta, r.dir.y), {z, z, r.dd[2].y}},
-                      {FMA(r.dd[2].z, delta, r.dir.z), {z, z, r.dd[2].z}}};
-    D3<float> sy{FMA(tD.d[2], delta, tD.v), {z, z, tD.d[2]}};
-    V3<D3<float>> Py = next_p_dual(Py0, Dy0, sy);
-    V3<D3<float>> Ny = optical_normal(F, Py);
-    V3<D3<float>> Dy = refracts ? refract(Dy0, Ny, n1.v, n2.v) : reflect(Dy0, Ny);
-    out.dd[2] = V3<float>{Dy.x.d[2], Dy.y.d[2], Dy.z.d[2]};
-  }
-  out.in_medium = refracts ? !r.in_medium : r.in_medium;
-  out.ignore_tri = i;
-  out.status = SPLAT_RAY_ACTIVE;
-  return out;
-}
 
 __global__ __launch_bounds__(TILE) void k_evolve(splat_adray_t* __restrict__ rays, int64_t R,
                                                  const int64_t* __restrict__ d_count,
@@ -135,8 +18,7 @@ __global__ __launch_bounds__(TILE) void k_evolve(splat_adray_t* __restrict__ ray
     int32_t i = hit_idx[r];
     if (i < 1 || i > N) i = 1;  // an index outside tris reads as the miss sentinel
     FTriF F = load_ftri(tris + (i - 1));
-    float u = !__builtin_isnan(force_rand) ? force_rand
-              : (rndm ? rndm[r] : u01(seed, RNG_RNDM | (uint32_t)bounce, (uint64_t)ray.dest));
+    float u = bounce_random(force_rand, rndm, r, seed, bounce, ray.dest);
     RayF o = evolve_one(ray, i, F, u, first_diffuse, opt);
     store_rayf(rays + r, o);
     active_after = o.status == SPLAT_RAY_ACTIVE;
@@ -145,17 +27,6 @@ __global__ __launch_bounds__(TILE) void k_evolve(splat_adray_t* __restrict__ ray
     int c = __syncthreads_count(active_after);
     if (threadIdx.x == 0) tile_counts[blockIdx.x] = c;
   }
-}
-
-// ---- final_evolution + atomic splat (src/ray_imagers.jl:4-37, :78-113) ---------------------
-__device__ __forceinline__ float curve_eval(const splat_lightmap_params_t& lm, int c, float lam) {
-  int n = lm.curve_n[c];
-  int m = 0;
-  while (m < n - 2 && lam >= lm.curve_x[c][m + 1]) ++m;
-  float x0 = lm.curve_x[c][m], x1 = lm.curve_x[c][m + 1];
-  float y0 = lm.curve_y[c][m], y1 = lm.curve_y[c][m + 1];
-  float slope = (y1 - y0) / (x1 - x0);
-  return FMA(slope, lam - x1, y1);
 }
 
 __global__ __launch_bounds__(256) void k_light_map(splat_adray_t* __restrict__ rays, int64_t R,
@@ -169,57 +40,7 @@ __global__ __launch_bounds__(256) void k_light_map(splat_adray_t* __restrict__ r
   RayF r = load_rayf(rays + rr);
   int32_t i = hit_idx[rr];
   if (i < 1 || i > N) i = 1;  // an index outside tris reads as the miss sentinel
-  const FTriF F = load_ftri(tris + (i - 1));
-  // final_evolution
-  D3<float> tD;
-  bool hit = get_hit_ad(r, F, i, &tD);
-  if (hit) {
-    V3<D3<float>> P = next_p_dual(pos_dual(r), dir_dual(r), tD);
-    split(P, &r.pos, r.dp);
-  } else {
-    r.pos = V3<float>{NAN, NAN, NAN};
-    r.dp[0] = r.dp[1] = r.dp[2] = r.pos;
-  }
-  r.ignore_tri = hit ? i : 1;
-  store_rayf(rays + rr, r);
-  // splat (k, dy, dx) in the reference's thread order
-  const int nl = lm.n_lambda > 0 ? lm.n_lambda : 1;
-  const float W = (float)lm.width, H = (float)lm.height;
-  const int64_t plane = (int64_t)lm.width * lm.height;
-  for (int k = 0; k < nl; ++k) {
-    const float lam = lm.n_lambda > 0 ? lm.spectrum[k] : r.lambda;
-    for (int iy = 0; iy < lm.n_delta; ++iy) {
-      for (int ix = 0; ix < lm.n_delta; ++ix) {
-        float xs = r.x + lm.delta[ix], ys = r.y + lm.delta[iy];
-        float dl = lam - r.lambda, dx = xs - r.x, dy = ys - r.y;
-        V3<float> p{FMA(r.dp[2].x, dy, FMA(r.dp[1].x, dx, FMA(r.dp[0].x, dl, r.pos.x))),
-                    FMA(r.dp[2].y, dy, FMA(r.dp[1].y, dx, FMA(r.dp[0].y, dl, r.pos.y))),
-                    FMA(r.dp[2].z, dy, FMA(r.dp[1].z, dx, FMA(r.dp[0].z, dl, r.pos.z)))};
-        V3<float> d{FMA(r.dd[2].x, dy, FMA(r.dd[1].x, dx, FMA(r.dd[0].x, dl, r.dir.x))),
-                    FMA(r.dd[2].y, dy, FMA(r.dd[1].y, dx, FMA(r.dd[0].y, dl, r.dir.y))),
-                    FMA(r.dd[2].z, dy, FMA(r.dd[1].z, dx, FMA(r.dd[0].z, dl, r.dir.z)))};
-        float bu, bv;
-        reverse_uv(F, p, &bu, &bv);
-        float w = (1.0f - bu) - bv;
-        float U = FMA(F.v[9].x, w, FMA(F.v[8].x, bv, F.v[7].x * bu));
-        float V = FMA(F.v[9].y, w, FMA(F.v[8].y, bv, F.v[7].y * bu));
-        if (!__builtin_isfinite(U) || !__builtin_isfinite(V)) continue;
-        float fi = ceilf(U * W), fj = ceilf(V * H);
-        fi = fi < 1.0f ? 1.0f : (fi > W ? W : fi);
-        fj = fj < 1.0f ? 1.0f : (fj > H ? H : fj);
-        const int64_t base = ((int64_t)fi - 1) + (int64_t)lm.width * ((int64_t)fj - 1);
-        V3<float> nrm = optical_normal(F, p);
-        float I = fabsf(dot3(nrm, normalize3(d)));
-        if (lm.weight_mode == SPLAT_WEIGHTS_IDENTITY) {
-          unsafeAtomicAdd(tex + base + plane * k, I);
-        } else if (lm.weight_mode == SPLAT_WEIGHTS_TABLE) {
-          for (int c = 0; c < lm.channels; ++c) unsafeAtomicAdd(tex + base + plane * c, I * lm.weights[k * 3 + c]);
-        } else {
-          for (int c = 0; c < lm.channels; ++c) unsafeAtomicAdd(tex + base + plane * c, I * curve_eval(lm, c, lam));
-        }
-      }
-    }
-  }
+  final_evolve_and_splat(r, i, load_ftri(tris + (i - 1)), lm, tex, rays + rr);
 }
 
 // ---- pinhole ray source: camera_ray (src/ray_generators.jl:18-50), evaluated in Float64 and
@@ -227,56 +48,7 @@ __global__ __launch_bounds__(256) void k_light_map(splat_adray_t* __restrict__ r
 __global__ __launch_bounds__(256) void k_raygen(const splat_pinhole_t cam, splat_adray_t* __restrict__ rays, int64_t R) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= R) return;
-  const int64_t H = cam.height, Wd = cam.width, L = cam.samples_per_pixel;
-  const int64_t P = q / L, l = q % L;
-  int64_t x, y;
-  if (cam.order == SPLAT_ORDER_TILED8 && H % 8 == 0 && Wd % 8 == 0) {
-    int64_t tile = P / 64, w = P % 64, tx = tile % (H / 8), ty = tile / (H / 8);
-    x = tx * 8 + (w % 8) + 1;
-    y = ty * 8 + (w / 8) + 1;
-  } else {
-    x = P % H + 1;
-    y = P / H + 1;
-  }
-  const int64_t dest = ((y - 1) * H + (x - 1)) * L + l + 1;
-  float lam = cam.lambda_min;
-  const float span = cam.lambda_max - cam.lambda_min;
-  if (cam.lambda_mode == SPLAT_LAMBDA_UNIFORM) {
-    lam = FMA(u01(cam.seed, RNG_LAMBDA, (uint64_t)dest), span, cam.lambda_min);
-  } else if (cam.lambda_mode == SPLAT_LAMBDA_STRATIFIED) {
-    float u = u01(cam.seed, RNG_LAMBDA, (uint64_t)dest);
-    lam = FMA(((float)l + u) / (float)L, span, cam.lambda_min);
-  }
-  float jit[3] = {0.f, 0.f, 0.f};
-  if (cam.jitter != 0.f) {
-    float mx = (float)(H > Wd ? H : Wd);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) jit[k] = (u01(cam.seed, RNG_JITTER | (uint64_t)k, (uint64_t)dest) * cam.jitter) / mx;
-  }
-  const double scale = ((double)H * (1.0 / __builtin_sqrt(2.0))) / (double)cam.fov_half_sin;
-  D3<double> X{(double)x, {0.0, 1.0, 0.0}}, Y{(double)y, {0.0, 0.0, 1.0}};
-  D3<double> xd = (X - (double)H / 2.0) / scale;
-  D3<double> yd = (Y - (double)Wd / 2.0) / scale;
-  D3<double> zd = SQRT((1.0 - xd * xd) - yd * yd);
-  splat_adray_t o;
-  for (int k = 0; k < 3; ++k) {
-    D3<double> dk = ((xd * (double)cam.right[k] + yd * (double)cam.up[k]) + zd * (double)cam.dir[k]) + (double)jit[k];
-    o.pos[k] = cam.pos[k];
-    o.dir[k] = (float)dk.v;
-    o.dddx[k] = (float)dk.d[1];
-    o.dddy[k] = (float)dk.d[2];
-    o.dpdl[k] = 0.f; o.dpdx[k] = 0.f; o.dpdy[k] = 0.f; o.dddl[k] = 0.f;
-  }
-  o.in_medium = 0;
-  for (int k = 0; k < 7; ++k) o._pad0[k] = 0;
-  o.ignore_tri = 1;
-  o.dest = dest;
-  o.lambda = lam;
-  o.x = 0.f;
-  o.y = 0.f;
-  o.status = SPLAT_RAY_ACTIVE;
-  o._pad1[0] = o._pad1[1] = o._pad1[2] = 0;
-  rays[q] = o;
+  raygen_one(cam, q, rays + q);
 }
 
 // ---- small utilities (graph-capturable: plain kernels, no memset nodes) ----------------------

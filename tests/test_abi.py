@@ -38,6 +38,12 @@ def test_layout_sizes_match_julia_isbits():
         [136, 52, 120, 24, 16]
 
 
+def test_param_struct_sizes_match_ctypes():
+    out = (C.c_int64 * 6)()
+    assert L.lib().splat_param_sizes(C.cast(out, C.c_void_p)) == 0
+    assert list(out) == [C.sizeof(t) for t in abi.PARAM_STRUCTS]
+
+
 def test_strerror_and_version():
     lib = L.lib()
     assert lib.splat_abi_version() == 1

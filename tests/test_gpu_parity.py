@@ -123,8 +123,11 @@ def test_next_hit_d_count_limits_rays(c1):
     assert (got[:1000] == want).all() and (got[1000:] == -7).all()
 
 
-@pytest.mark.parametrize("depth", [0, 2, 4, 6])
-def test_next_hit_bv_equals_bf(depth):
+@pytest.mark.parametrize("build,depth,group_size", [
+    ("reference", 0, 0), ("reference", 2, -1), ("reference", 4, 0), ("reference", 6, 3),
+    ("object", 6, -1), ("object", 8, 0), ("object", 8, 5),
+])
+def test_next_hit_bv_equals_bf(build, depth, group_size):
     sc = H.small_c2(3)
     cam = scene.camera_c2(seed=3, height=64, width=64)
     rays = O.raygen(cam)
@@ -132,7 +135,9 @@ def test_next_hit_bv_equals_bf(depth):
     rays = np.concatenate([rays, H.rays_towards(pts, seed=13), H.random_rays(1000, seed=14, n_tris=sc.n_tris)])
     want, _ = O.next_hit(rays, sc.ntris())
     tris_d = api.to_device(sc.ftris)
-    hitter = api.HIPBVHitter(tris_d, host_tris=sc.ftris, depth=depth)
+    part = bv.bv_partition if build == "reference" else bv.object_partition
+    bvs, mem = part(sc.ftris, depth)
+    hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=group_size)
     rays_d = api.to_device(rays)
     tr = api.HIPTracer(rays_d)
     api.next_hit_(tr, hitter, rays_d)
