@@ -96,12 +96,68 @@ def cpu_baseline(sc, cam, bvmem, params, lm, budget_s: float = 12.0):
                       f"frame (same BV culling, 4 bounces, splat), OpenMP {threads} threads"}
 
 
+def batched_vs_plain(reps: int = 3):
+    """64 mixed frames (256^2-1024^2 rays, icosphere subdiv 2-5 + plane, 4 bounces): all in one
+    engine batch (shared launches, persistent work-queue hitter, graph-replayed steps) versus the
+    same frames traced one frame per launch sequence."""
+    import torch
+
+    from splat_amd import api, serve
+
+    scenes, reqs = serve.batched_workload(64)
+    rays = sum(r.rays for r in reqs)
+    frames = [api.Frame(r.cam, scenes[r.scene].hitter, scenes[r.scene].tris, r.params, r.lm) for r in reqs]
+    for f in frames:  # warm-up
+        f.trace()
+    torch.cuda.synchronize()
+    plain = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        for f in frames:
+            f.trace()
+        torch.cuda.synchronize()
+        plain.append(time.perf_counter() - t)
+    del frames
+    arena = int(sum(r.rays for r in reqs) * 440 * 1.05) + (1 << 28)
+    eng = api.Engine(arena, max_slots=64)
+    batched = []
+    for rep in range(reps + 1):
+        for r in reqs:
+            r.done = float("nan")
+        st = serve.run_requests(eng, scenes, reqs, realtime=False, max_pending_steps=8)
+        if rep:
+            batched.append(st.wall_s)
+    eng.close()
+    tp, tb = min(plain), min(batched)
+    return {"frames": len(reqs), "rays": rays, "plain_mrays_per_s": round(rays / tp / 1e6, 3),
+            "batched_mrays_per_s": round(rays / tb / 1e6, 3), "batched_vs_plain": round(tp / tb, 3),
+            "plain_s": round(tp, 4), "batched_s": round(tb, 4), "steps_per_batch": st.steps}
+
+
+def served(n_requests: int = 100, rate: float = 12.0):
+    """Poisson arrivals over 8 resident scenes (320 to ~178k triangles), resolution uniform over
+    256^2-2048^2, 1-16 stratified wavelengths per pixel, 4 bounces, admitted against an HBM budget."""
+    from splat_amd import api, serve
+
+    scenes = serve.serving_scenes()
+    reqs = serve.serving_requests(n_requests, rate, len(scenes), [s.sc.first_diffuse for s in scenes])
+    eng = api.Engine(96 << 30, max_slots=64)
+    st = serve.run_requests(eng, scenes, reqs, realtime=True)
+    eng.close()
+    out = st.summary()
+    out.update({"arrival_rate_per_s": rate, "hbm_budget_gib": 96,
+                "triangles": [s.sc.n_tris for s in scenes]})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -165,6 +221,11 @@ def main() -> int:
     flop_per_launch = (FLOP_PER_BOX_TEST * box_tests + FLOP_PER_TRI_TEST * alg_tests) / n_launch
     achieved = flop_per_launch / avg_launch_s / 1e12
 
+    del frame
+    torch.cuda.empty_cache()
+    batched = None if args.no_batched else batched_vs_plain()
+    torch.cuda.empty_cache()
+    serve_out = None if args.no_served else served()
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, bvmem, params, lm)
     out = {
         "metric": METRIC,
@@ -208,6 +269,8 @@ def main() -> int:
             "tri_tests_exec_per_frame": exec_tests,
             "simd_efficiency": round(alg_tests / max(exec_tests, 1), 4),
         },
+        "batched": batched,
+        "served": serve_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -332,6 +332,8 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream);
  * launched step is done). Their slots and arena space are released. */
 int32_t splat_engine_poll(splat_engine_t* e, int32_t wait, int64_t* tags, int32_t max_tags, int32_t* n_out);
 int32_t splat_engine_in_flight(const splat_engine_t* e);
+/* Blocks until at most `keep` launched steps are still running on the device. */
+int32_t splat_engine_sync_steps(splat_engine_t* e, int32_t keep);
 int64_t splat_engine_arena_free(const splat_engine_t* e);
 /* on != 0 (default): each step replays one captured HIP graph; 0: plain stream launches. */
 int32_t splat_engine_use_graph(splat_engine_t* e, int32_t on);

@@ -289,6 +289,10 @@ class Engine:
             self._keep.pop(t, None)
         return tags
 
+    def sync_steps(self, keep: int) -> None:
+        """Block until at most `keep` launched steps are still running."""
+        L.check(self._lib.splat_engine_sync_steps(self.h, int(keep)), "engine_sync_steps")
+
     @property
     def in_flight(self) -> int:
         return int(self._lib.splat_engine_in_flight(self.h))

@@ -1,0 +1,110 @@
+"""Autotuner for the hitter launch shapes.
+
+Candidates are timed with HIP events on the caller's real scene and rays (through the C ABI),
+the fastest is kept, and choices are cached in memory and, optionally, in a JSON file keyed by
+(kernel, log2 ray count, log2 triangle count), so a serving process tunes each shape once.
+
+Tuned knobs:
+  brute force -- waves per workgroup W that split the triangle list (1, 2, 4, 8, 16);
+  BV          -- leaf count of the object-median build (2^d) and the coarse group size
+                 (none, ~sqrt(leaves), or a fixed size).
+All candidates return identical hit indices (the tuner asserts it on the tuning rays).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+from . import api, bv
+
+_cache: dict[str, dict] = {}
+
+
+def _key(kind: str, R: int, N: int) -> str:
+    return f"{kind}:r{int(math.log2(max(R, 1)))}:n{int(math.log2(max(N, 1)))}"
+
+
+def _load(path: str | None) -> None:
+    if path and os.path.exists(path):
+        with open(path) as f:
+            _cache.update(json.load(f))
+
+
+def _save(path: str | None) -> None:
+    if path:
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(_cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+
+
+def _time_hits(hitter, rays, reps: int = 5) -> tuple[float, np.ndarray]:
+    import torch
+
+    tr = api.HIPTracer(rays)
+    api.next_hit_(tr, hitter, rays)  # warm-up
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        api.next_hit_(tr, hitter, rays)
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    return float(np.median(ts)), tr.hit_idx.cpu().numpy()
+
+
+def tune_bf(n_tris_dev, rays, cache_path: str | None = None) -> dict:
+    """Best W for the brute-force hitter on these rays; returns {"waves": W, "ms": t}."""
+    _load(cache_path)
+    key = _key("bf", int(rays.shape[0]), int(n_tris_dev.shape[0]))
+    if key in _cache:
+        return _cache[key]
+    best, ref = None, None
+    for w in (1, 2, 4, 8, 16):
+        h = api.HIPBruteForceHitter(n_tris_dev, tune_waves=w)
+        ms, hits = _time_hits(h, rays)
+        if ref is None:
+            ref = hits
+        elif not np.array_equal(ref, hits):
+            raise RuntimeError(f"brute-force candidate W={w} changed the hit indices")
+        if best is None or ms < best["ms"]:
+            best = {"waves": w, "ms": ms}
+    _cache[key] = best
+    _save(cache_path)
+    return best
+
+
+def tune_bv(tris_dev, host_tris: np.ndarray, rays, cache_path: str | None = None,
+            depths: tuple[int, ...] | None = None, group_sizes: tuple[int, ...] = (-1, 0, 8, 32)) -> dict:
+    """Best (leaf depth, group size) for the BV hitter on these rays; returns a dict with
+    "depth", "group_size", "ms" and builds the winning hitter under "hitter" (not cached)."""
+    _load(cache_path)
+    N = int(host_tris.shape[0])
+    key = _key("bv", int(rays.shape[0]), N)
+    if depths is None:
+        d0 = max(2, int(round(math.log2(max(N, 2) / 40))))
+        depths = tuple(d for d in (d0 - 1, d0, d0 + 1) if d >= 1)
+    if key not in _cache:
+        best, ref = None, None
+        for d in depths:
+            bvs, mem = bv.object_partition(host_tris, d)
+            for g in group_sizes:
+                h = api.HIPBVHitter(tris_dev, bvs, mem, group_size=g)
+                ms, hits = _time_hits(h, rays)
+                if ref is None:
+                    ref = hits
+                elif not np.array_equal(ref, hits):
+                    raise RuntimeError(f"BV candidate depth={d} group={g} changed the hit indices")
+                if best is None or ms < best["ms"]:
+                    best = {"depth": d, "group_size": g, "ms": ms}
+        _cache[key] = best
+        _save(cache_path)
+    choice = dict(_cache[key])
+    bvs, mem = bv.object_partition(host_tris, choice["depth"])
+    choice["hitter"] = api.HIPBVHitter(tris_dev, bvs, mem, group_size=choice["group_size"])
+    return choice

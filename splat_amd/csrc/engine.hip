@@ -588,6 +588,13 @@ int32_t splat_engine_poll(splat_engine_t* e, int32_t wait, int64_t* tags, int32_
   return SPLAT_OK;
 }
 
+int32_t splat_engine_sync_steps(splat_engine_t* e, int32_t keep) {
+  if (!e || keep < 0) return SPLAT_ERR_INVALID_ARG;
+  const int32_t n = (int32_t)e->pending.size();
+  if (n > keep && hipEventSynchronize(e->pending[(size_t)(n - keep - 1)].ev) != hipSuccess) return SPLAT_ERR_HIP;
+  return SPLAT_OK;
+}
+
 int32_t splat_engine_in_flight(const splat_engine_t* e) {
   if (!e) return 0;
   int32_t n = 0;

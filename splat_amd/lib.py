@@ -68,6 +68,7 @@ SIGNATURES = {
     "splat_engine_step": (i32, [P, P]),
     "splat_engine_poll": (i32, [P, i32, P, i32, P]),
     "splat_engine_in_flight": (i32, [P]),
+    "splat_engine_sync_steps": (i32, [P, i32]),
     "splat_engine_arena_free": (i64, [P]),
     "splat_engine_use_graph": (i32, [P, i32]),
 }
