@@ -233,9 +233,14 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t n_tris_len, in
 /* next_hit!(tracer, hitter, rays, n_tris) (src/hitters.jl:24-30, :34-82, :168-226, :615-648):
  * hit_idx[r] = the lexicographic minimum over triangles j of (t_rj, j), ties to the lowest
  * index, 1 on a miss. `d_count` (device int64, nullable) limits the rays to the first
- * min(*d_count, R) -- the live count kept on the device between bounces. */
+ * min(*d_count, R) -- the live count kept on the device between bounces. A BV hit set needs a
+ * device workspace of splat_next_hit_ws_bytes(R) bytes (the packed (t, index) key per ray
+ * that the reference encodes in src/atomic_argmin.jl:5-20);
+ * brute force takes ws = NULL. */
+size_t splat_next_hit_ws_bytes(int64_t R);
 int32_t splat_next_hit(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
-                       const splat_hitset_t* hs, int32_t* hit_idx, void* stream);
+                       const splat_hitset_t* hs, int32_t* hit_idx, void* ws, size_t ws_bytes,
+                       void* stream);
 /* One-call brute force over an n_tris array (prepares into ws, then hits). */
 int32_t splat_next_hit_bf(const splat_adray_t* rays, int64_t R, const splat_ntri_t* n_tris,
                           int32_t n_tris_len, int32_t* hit_idx, void* ws, size_t ws_bytes,
@@ -331,7 +336,11 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream);
 /* Reports the tags of frames whose final step has completed (wait != 0: block until every
  * launched step is done). Their slots and arena space are released. */
 int32_t splat_engine_poll(splat_engine_t* e, int32_t wait, int64_t* tags, int32_t max_tags, int32_t* n_out);
+/* Admitted frames not yet reported by splat_engine_poll. */
 int32_t splat_engine_in_flight(const splat_engine_t* e);
+/* Frames the next splat_engine_step advances (in flight minus those whose last step has been
+ * launched): 0 means a step would launch nothing and the caller should poll with wait != 0. */
+int32_t splat_engine_runnable(const splat_engine_t* e);
 /* Blocks until at most `keep` launched steps are still running on the device. */
 int32_t splat_engine_sync_steps(splat_engine_t* e, int32_t keep);
 int64_t splat_engine_arena_free(const splat_engine_t* e);

@@ -79,11 +79,24 @@ function HIPBVHitter(tris::ROCArray{FTri}, bvs, memberships; group_size = 0)
     HIPBVHitter(hs, ws)
 end
 
+# device workspace of splat_next_hit (the packed (t, index) key per ray), one per tracer
+const HIT_WS = IdDict{Any,ROCArray{UInt8}}()
+function hit_workspace(tracer, R)
+    need = ccall((:splat_next_hit_ws_bytes, LIB), Csize_t, (Int64,), R)
+    ws = get(HIT_WS, tracer, nothing)
+    if ws === nothing || length(ws) < need
+        ws = HIT_WS[tracer] = ROCArray{UInt8}(undef, need)
+    end
+    return ws
+end
+
 # next_hit!(tracer, hitter, rays, n_tris) -- src/hitters.jl:24-30 semantics
 function next_hit!(tracer, hitter::Union{HIPBruteForceHitter,HIPBVHitter}, rays::ROCArray{ADRay}, n_tris)
+    ws = hit_workspace(tracer, length(rays))  # ROCArray{UInt8} of splat_next_hit_ws_bytes(R), kept by the tracer
     check(ccall((:splat_next_hit, LIB), Int32,
-                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{HitSet}, Ptr{Int32}, Ptr{Cvoid}),
-                pointer(rays), length(rays), C_NULL, hitter.hs, pointer(tracer.hit_idx), stream_ptr()), "next_hit")
+                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{HitSet}, Ptr{Int32}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+                pointer(rays), length(rays), C_NULL, hitter.hs, pointer(tracer.hit_idx), pointer(ws), length(ws),
+                stream_ptr()), "next_hit")
 end
 
 # partition!(rays, swap; by = r.status != ACTIVE) -- src/partition.jl:122-132, stable

@@ -121,12 +121,18 @@ class HIPTracer:
         self.N = N
         self._run_ws = None
         self._part_ws = None
+        self._hit_ws = None
         self.d_count = torch.zeros(1, dtype=torch.int64, device=rays.device)
 
     def run_ws(self, R: int):
         if self._run_ws is None or self._run_ws.numel() < L.lib().splat_run_ws_bytes(R):
             self._run_ws = _ws(L.lib().splat_run_ws_bytes(R))
         return self._run_ws
+
+    def hit_ws(self, R: int):
+        if self._hit_ws is None or self._hit_ws.numel() < L.lib().splat_next_hit_ws_bytes(R):
+            self._hit_ws = _ws(L.lib().splat_next_hit_ws_bytes(R))
+        return self._hit_ws
 
     def part_ws(self, R: int):
         if self._part_ws is None or self._part_ws.numel() < L.lib().splat_partition_ws_bytes(R):
@@ -139,8 +145,10 @@ class HIPTracer:
 def next_hit_(tracer: HIPTracer, hitter, rays, n_tris=None, d_count=None) -> None:
     """next_hit!(tracer, hitter, rays, n_tris): tracer.hit_idx[r] <- nearest hit index (1 = miss)."""
     _check_rows(rays, abi.ADRAY.itemsize, "rays")
-    L.check(L.lib().splat_next_hit(L.ptr(rays), _nrows(rays), L.ptr(d_count), L.ref(hitter.hs), L.ptr(tracer.hit_idx),
-                                   L.stream_handle()), "next_hit")
+    R = _nrows(rays)
+    ws = tracer.hit_ws(R)
+    L.check(L.lib().splat_next_hit(L.ptr(rays), R, L.ptr(d_count), L.ref(hitter.hs), L.ptr(tracer.hit_idx),
+                                   L.ptr(ws), ws.numel(), L.stream_handle()), "next_hit")
 
 
 def evolve_(rays, hit_idx, tris, *, first_diffuse: int, rndm=None, force_rand=None, seed: int = 0, bounce: int = 1,
@@ -296,6 +304,11 @@ class Engine:
     @property
     def in_flight(self) -> int:
         return int(self._lib.splat_engine_in_flight(self.h))
+
+    @property
+    def runnable(self) -> int:
+        """Frames the next step advances; 0 with frames in flight: only completions are pending."""
+        return int(self._lib.splat_engine_runnable(self.h))
 
     @property
     def arena_free(self) -> int:

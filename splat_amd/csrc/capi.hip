@@ -36,7 +36,7 @@ constexpr int MAX_DEPTH = 64;
 struct RunWs {
   splat_adray_t* s1;
   splat_adray_t* s2;
-  int32_t* hit_scratch;
+  uint64_t* keys;  // packed (t, index) nearest hit of each live ray
   int32_t* tile_counts;
   int32_t* tile_offsets;
   int64_t* counters;  // counters[k] = live rays after bounce k (counters[0] = R)
@@ -49,7 +49,7 @@ RunWs carve_run(void* ws, int64_t R) {
   RunWs w;
   w.s1 = c.take<splat_adray_t>((size_t)R);
   w.s2 = c.take<splat_adray_t>((size_t)R);
-  w.hit_scratch = c.take<int32_t>((size_t)R);
+  w.keys = c.take<uint64_t>((size_t)R);
   w.tile_counts = c.take<int32_t>((size_t)n_tiles);
   w.tile_offsets = c.take<int32_t>((size_t)n_tiles);
   w.counters = c.take<int64_t>(MAX_DEPTH + 2);
@@ -84,16 +84,59 @@ int32_t check_hitset(const splat_hitset_t* hs) {
   return SPLAT_OK;
 }
 
-void hit_dispatch(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs, int32_t* hit_idx,
-                  hipStream_t s) {
+BVScene bv_scene(const splat_hitset_t* hs) {
+  BVScene b;
+  b.boxes = static_cast<const float4*>(hs->d_boxes);
+  b.ranges = static_cast<const int2*>(hs->d_ranges);
+  b.recs = static_cast<const TriRec*>(hs->d_recs);
+  b.n_bv = hs->n_bv;
+  const bool g = hs->d_groups && hs->group_size > 0 && hs->n_groups > 0;
+  b.groups = g ? static_cast<const float4*>(hs->d_groups) : nullptr;
+  b.gsize = g ? hs->group_size : 0;
+  b.n_groups = g ? hs->n_groups : 0;
+  return b;
+}
+
+int bf_waves(const splat_hitset_t* hs, int64_t R) {
+  return hs->tune_waves > 0 ? hs->tune_waves : choose_bf_waves(R, hs->n_tris);
+}
+
+// nearest hit of the live rays as packed (t, index) keys: hit_keys_init, then hit_keys_launch
+void hit_keys_init(int64_t R, const int64_t* d_count, const splat_hitset_t* hs, uint64_t* keys, hipStream_t s) {
+  if (hs->kind == SPLAT_HITTER_BV) launch_key_init(keys, R, d_count, s);  // brute force writes every key
+}
+void hit_keys_launch(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs,
+                     uint64_t* keys, hipStream_t s) {
+  if (hs->kind == SPLAT_HITTER_BF)
+    launch_hit_bf(rays, R, d_count, static_cast<const TriRec*>(hs->d_recs), hs->n_tris, nullptr, keys, bf_waves(hs, R),
+                  hs->d_stats, s);
+  else
+    launch_hit_bv(rays, R, d_count, bv_scene(hs), keys, hs->d_stats, s);
+}
+
+// nearest hit index of the live rays into hit_idx (keys: scratch of the BV hitter)
+void hit_index(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs, int32_t* hit_idx,
+               uint64_t* keys, hipStream_t s) {
   if (hs->kind == SPLAT_HITTER_BF) {
-    int w = hs->tune_waves > 0 ? hs->tune_waves : choose_bf_waves(R, hs->n_tris);
-    launch_hit_bf(rays, R, d_count, static_cast<const TriRec*>(hs->d_recs), hs->n_tris, hit_idx, w, hs->d_stats, s);
+    launch_hit_bf(rays, R, d_count, static_cast<const TriRec*>(hs->d_recs), hs->n_tris, hit_idx, nullptr, bf_waves(hs, R),
+                  hs->d_stats, s);
   } else {
-    launch_hit_bv(rays, R, d_count, static_cast<const float4*>(hs->d_boxes), static_cast<const int2*>(hs->d_ranges),
-                  static_cast<const TriRec*>(hs->d_recs), hs->n_bv, static_cast<const float4*>(hs->d_groups),
-                  hs->d_groups ? hs->group_size : 0, hs->d_groups ? hs->n_groups : 0, hit_idx, hs->d_stats, s);
+    hit_keys_init(R, d_count, hs, keys, s);
+    hit_keys_launch(rays, R, d_count, hs, keys, s);
+    launch_key_final(keys, R, d_count, hit_idx, s);
   }
+}
+
+struct HitWs {
+  uint64_t* keys;
+  size_t bytes;
+};
+HitWs carve_hit(void* ws, int64_t R) {
+  Carver c(ws);
+  HitWs w;
+  w.keys = c.take<uint64_t>((size_t)R);
+  w.bytes = c.used;
+  return w;
 }
 
 int32_t check_optics(const splat_optics_t* o) {
@@ -135,21 +178,31 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
   splat_adray_t* nxt = w.s2;
   for (int k = 1; k <= p->depth; ++k) {
     const int64_t* live = &w.counters[k - 1];
+    hit_keys_init(R, live, hs, w.keys, s);
     mark(2 * (k - 1));
-    hit_dispatch(cur, R, live, hs, w.hit_scratch, s);
+    hit_keys_launch(cur, R, live, hs, w.keys, s);
     mark(2 * (k - 1) + 1);
-    launch_evolve(cur, R, live, w.hit_scratch, tris, hs->n_tris, nullptr, p->force_rand, p->seed, k, p->first_diffuse, p->optics,
+    launch_evolve(cur, R, live, nullptr, w.keys, tris, hs->n_tris, nullptr, p->force_rand, p->seed, k, p->first_diffuse, p->optics,
                   w.tile_counts, s);
     launch_scan_tiles(w.tile_counts, n_tiles, live, R, w.tile_offsets, &w.counters[k], s);
-    launch_scatter(cur, R, live, w.tile_offsets, &w.counters[k], nxt, final_rays, w.hit_scratch, hit_final, s);
+    launch_scatter(cur, R, live, w.tile_offsets, &w.counters[k], nxt, final_rays, nullptr, w.keys, hit_final, s);
     splat_adray_t* t = cur;
     cur = nxt;
     nxt = t;
   }
   const int64_t* live = &w.counters[p->depth];
-  mark(2 * p->depth);
-  hit_dispatch(cur, R, live, hs, hit_final, s);  // the reference's realigning next_hit! (src/tracer.jl:280)
-  mark(2 * p->depth + 1);
+  // the reference's realigning next_hit! (src/tracer.jl:280)
+  if (hs->kind == SPLAT_HITTER_BF) {
+    mark(2 * p->depth);
+    hit_index(cur, R, live, hs, hit_final, nullptr, s);
+    mark(2 * p->depth + 1);
+  } else {
+    hit_keys_init(R, live, hs, w.keys, s);
+    mark(2 * p->depth);
+    hit_keys_launch(cur, R, live, hs, w.keys, s);
+    mark(2 * p->depth + 1);
+    launch_key_final(w.keys, R, live, hit_final, s);
+  }
   launch_copy_rays(cur, final_rays, R, live, s);
   return launch_status();
 }
@@ -298,11 +351,19 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   return SPLAT_OK;
 }
 
+size_t splat_next_hit_ws_bytes(int64_t R) { return R >= 0 ? carve_hit(nullptr, R).bytes : 0; }
+
 int32_t splat_next_hit(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs,
-                       int32_t* hit_idx, void* stream) {
+                       int32_t* hit_idx, void* ws, size_t ws_bytes, void* stream) {
   if (R < 0 || (R > 0 && (!rays || !hit_idx))) return SPLAT_ERR_INVALID_ARG;
   if (int32_t e = check_hitset(hs)) return e;
-  hit_dispatch(rays, R, d_count, hs, hit_idx, S(stream));
+  HitWs w{nullptr, 0};
+  if (hs->kind == SPLAT_HITTER_BV) {
+    if (!ws) return SPLAT_ERR_INVALID_ARG;
+    if (ws_bytes < splat_next_hit_ws_bytes(R)) return SPLAT_ERR_WORKSPACE;
+    w = carve_hit(ws, R);
+  }
+  hit_index(rays, R, d_count, hs, hit_idx, w.keys, S(stream));
   return launch_status();
 }
 
@@ -310,7 +371,7 @@ int32_t splat_next_hit_bf(const splat_adray_t* rays, int64_t R, const splat_ntri
                           void* ws, size_t ws_bytes, void* stream) {
   splat_hitset_t hs;
   if (int32_t e = splat_hitset_prepare_bf(n_tris, N, ws, ws_bytes, &hs, stream)) return e;
-  return splat_next_hit(rays, R, nullptr, &hs, hit_idx, stream);
+  return splat_next_hit(rays, R, nullptr, &hs, hit_idx, nullptr, 0, stream);
 }
 
 int32_t splat_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
@@ -318,7 +379,7 @@ int32_t splat_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, con
                      int32_t bounce, int32_t first_diffuse, const splat_optics_t* optics, void* stream) {
   if (R < 0 || (R > 0 && (!rays || !hit_idx)) || !tris || N < 1) return SPLAT_ERR_INVALID_ARG;
   if (int32_t e = check_optics(optics)) return e;
-  launch_evolve(rays, R, d_count, hit_idx, tris, N, rndm, force_rand, seed, bounce, first_diffuse, *optics, nullptr, S(stream));
+  launch_evolve(rays, R, d_count, hit_idx, nullptr, tris, N, rndm, force_rand, seed, bounce, first_diffuse, *optics, nullptr, S(stream));
   return launch_status();
 }
 
@@ -338,7 +399,7 @@ int32_t splat_partition_active(splat_adray_t* rays, splat_adray_t* swap, int64_t
   } else {
     launch_count_active(rays, R, nullptr, w.tile_counts, s);
     launch_scan_tiles(w.tile_counts, n_tiles, nullptr, R, w.tile_offsets, total, s);
-    launch_scatter(rays, R, nullptr, w.tile_offsets, total, swap, swap, nullptr, nullptr, s);
+    launch_scatter(rays, R, nullptr, w.tile_offsets, total, swap, swap, nullptr, nullptr, nullptr, s);
     if (hipMemcpyAsync(rays, swap, (size_t)R * sizeof(splat_adray_t), hipMemcpyDeviceToDevice, s) != hipSuccess)
       return SPLAT_ERR_HIP;
   }

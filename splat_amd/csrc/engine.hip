@@ -5,10 +5,11 @@
 //   0 <= s < d  : hit (live rays) -> evolve -> per-frame scan -> stable scatter   (bounce s+1)
 //   s == d      : realigning hit -> copy of the live rays -> final evolution + splat -> free
 // A step is one fixed sequence of launches for every slot at once: a planning kernel turns the
-// device-side live counts into per-kind work-item prefixes; the hitter is persistent and pulls
-// 64-ray items of every frame from a device work queue (frames of any size and scene cost share
-// the chip); the other stages walk their items grid-stride. Launch shapes never change, so one
-// step is captured into a HIP graph once and replayed. The host tracks stages itself (every
+// device-side live counts into per-kind work-item prefixes (and the hitter's split factor); the
+// hitter runs one wave per (64-ray group, split) item of any frame, so frames of any size and
+// scene cost share the chip; the other stages walk their items grid-stride. Launch shapes depend
+// only on the hitter's grid size class, so each class is captured into a HIP graph once and
+// replayed. The host tracks stages itself (every
 // admitted frame advances exactly one stage per step), so it never reads device state to learn
 // that a frame has finished: an event recorded after each step tells when.
 #include <hip/hip_runtime.h>
@@ -35,7 +36,7 @@ struct SlotDev {
   splat_adray_t* F;        // final rays (the frame's output order)
   splat_adray_t* S[2];     // live rays, ping-pong by stage parity
   int32_t* hit_final;
-  int32_t* hit_scratch;
+  uint64_t* keys;          // packed (t, index) nearest hit of each live ray
   int32_t* tile_counts;
   int32_t* tile_offsets;
   float* tex;
@@ -56,7 +57,7 @@ struct EngineDev {
   int32_t* stage;      // -1 = empty
   int64_t* counters;   // [slot][MAXD + 2]: live rays after bounce k (k = 0: R)
   int32_t* pre;        // [kind][MAX_SLOTS + 1] exclusive prefixes of work items
-  int32_t* head;       // hitter work-queue head
+  int32_t* split;      // splits per 64-ray hitter item in this step
   int32_t n_slots;
 };
 
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
     if (t == 0) P[E.n_slots] = buf[E.n_slots - 1];
     __syncthreads();
   }
-  if (t == 0) *E.head = 0;
+  if (t == 0) *E.split = split_factor(E.pre[K_HIT * (MAX_SLOTS + 1) + E.n_slots], MAX_SPLIT);
 }
 
 __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
@@ -128,30 +129,39 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
     const int64_t b = (int64_t)(i - Q[s]) * CLEAR_ITEM;
     for (int64_t k = b + threadIdx.x; k < b + CLEAR_ITEM && k < D.tex_n; k += blockDim.x) D.tex[k] = 0.0f;
   }
+  // keys of every live ray <- (Inf, 1) before the hitter merges into them
+  const int32_t* H = E.pre + K_HIT * (MAX_SLOTS + 1);
+  constexpr int PER_BLOCK = 256 / HIT_ITEM;
+  for (int32_t i = blockIdx.x * PER_BLOCK + threadIdx.x / HIT_ITEM; i < H[n]; i += gridDim.x * PER_BLOCK) {
+    const int32_t s = item_slot(H, n, i);
+    const SlotDev& D = E.slots[s];
+    const int32_t st = E.stage[s];
+    const int64_t A = st == 0 ? D.R : E.counters[(int64_t)s * (MAXD + 2) + st];
+    const int64_t r = (int64_t)(i - H[s]) * HIT_ITEM + (threadIdx.x & (HIT_ITEM - 1));
+    if (r < A) D.keys[r] = KEY_MISS;
+  }
 }
 
-// persistent hitter: every wave pulls 64-ray items from the shared queue until it is empty
+// hitter: one wave per (64-ray group, split) item over every frame's live rays; the splits of a
+// group merge into its keys with a 64-bit atomic minimum. The grid (chosen by the host from the
+// frames' ray counts) covers the most items the live counts can need; waves past them leave.
 __global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_HIT * (MAX_SLOTS + 1);
-  const int32_t total = P[n];
-  const int lane = threadIdx.x & 63;
-  for (;;) {
-    int32_t mine = 0;
-    if (lane == 0) mine = atomicAdd(E.head, 1);
-    const int32_t i = __builtin_amdgcn_readfirstlane(mine);
-    if (i >= total) break;  // every wave leaves once the queue is drained
-    const int32_t s = item_slot(P, n, i);
+  const int32_t S = *E.split;
+  const int64_t total = (int64_t)P[n] * S;
+  HitStats hs;
+  const int64_t w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (w < total) {
+    const int32_t i = (int32_t)w;
+    const int32_t item = i / S;
+    const int32_t s = item_slot(P, n, item);
     const SlotDev& D = E.slots[s];
     const int32_t st = E.stage[s];
     const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
-    const int64_t r = (int64_t)(i - P[s]) * HIT_ITEM + lane;
-    const bool valid = r < A;
-    const RayLanes L = load_ray_lanes(D.S[st & 1], r, valid);
-    HitStats hs;
-    const int32_t bi = D.scene.groups ? bv_traverse<false, true>(L, valid, D.scene, hs)
-                                      : bv_traverse<false, false>(L, valid, D.scene, hs);
-    if (valid) (st == D.depth ? D.hit_final : D.hit_scratch)[r] = bi;
+    const int64_t base = (int64_t)(item - P[s]) * HIT_ITEM;
+    if (D.scene.groups) bv_item<false, true>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs);
+    else bv_item<false, false>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs);
   }
 }
 
@@ -169,7 +179,7 @@ __global__ __launch_bounds__(TILE) void k_b_evolve(EngineDev E) {
     bool active_after = false;
     if (r < A && rays[r].status == SPLAT_RAY_ACTIVE) {
       RayF ray = load_rayf(rays + r);
-      int32_t h = D.hit_scratch[r];
+      int32_t h = key_index(D.keys[r]);
       if (h < 1 || h > D.n_tris) h = 1;
       const FTriF F = load_ftri(D.tris + (h - 1));
       const float u = bounce_random(D.force_rand, nullptr, r, D.seed, st + 1, ray.dest);
@@ -227,7 +237,8 @@ __global__ __launch_bounds__(TILE) void k_b_scatter(EngineDev E) {
     const int nvalid = (int)(A - base < TILE ? A - base : TILE);
     const int64_t off_a = D.tile_offsets[tile];
     const int64_t off_r = C[st + 1] + (base - off_a);
-    scatter_tile(D.S[st & 1] + base, nvalid, off_a, off_r, D.S[(st + 1) & 1], D.F, D.hit_scratch + base, D.hit_final);
+    scatter_tile(D.S[st & 1] + base, nvalid, off_a, off_r, D.S[(st + 1) & 1], D.F, nullptr, D.hit_final,
+                 D.keys + base);
   }
 }
 
@@ -246,6 +257,7 @@ __global__ __launch_bounds__(TILE) void k_b_final(EngineDev E) {
     const uint2* src = reinterpret_cast<const uint2*>(D.S[st & 1] + base);
     uint2* dst = reinterpret_cast<uint2*>(D.F + base);
     for (int64_t k = threadIdx.x; k < nw; k += TILE) dst[k] = src[k];
+    if (base + threadIdx.x < A) D.hit_final[base + threadIdx.x] = key_index(D.keys[base + threadIdx.x]);
   }
 }
 
@@ -319,6 +331,7 @@ struct HostSlot {
   bool used = false;
   int32_t stage = -1;  // host mirror: stage the slot runs at in the next step
   int32_t depth = 0;
+  int64_t R = 0;
   int64_t tag = 0;
   std::vector<Block> blocks;
 };
@@ -330,7 +343,7 @@ struct Pending {
 
 size_t frame_bytes(int64_t R, bool with_outputs) {
   int64_t tiles = (R + TILE - 1) / TILE;
-  size_t b = 2 * al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 4) + 2 * al((size_t)tiles * 4);
+  size_t b = 2 * al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 8) + 2 * al((size_t)tiles * 4);
   if (with_outputs) b += al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 4);
   return b;
 }
@@ -351,20 +364,19 @@ struct splat_engine {
   std::vector<HostSlot> slots;
   std::deque<Pending> pending;
   std::vector<hipEvent_t> free_events;
-  hipGraphExec_t graph = nullptr;
-  hipStream_t cap_stream = nullptr;  // the graph is captured here (the null stream cannot capture)
+  std::map<int64_t, hipGraphExec_t> graphs;  // one captured step per hitter grid size
+  hipStream_t cap_stream = nullptr;  // graphs are captured here (the null stream cannot capture)
   int use_graph = 1;
   int grid = 1024;
-  int hit_grid = 2048;
 };
 
 namespace {
 
-int32_t launch_step_kernels(splat_engine* e, hipStream_t s) {
+int32_t launch_step_kernels(splat_engine* e, int64_t hit_blocks, hipStream_t s) {
   const EngineDev& E = e->dev;
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
   hipLaunchKernelGGL(k_b_raygen, dim3(e->grid), dim3(256), 0, s, E);
-  hipLaunchKernelGGL(k_b_hit, dim3(e->hit_grid), dim3(256), 0, s, E);
+  hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_evolve, dim3(e->grid), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
   hipLaunchKernelGGL(k_b_scatter, dim3(e->grid), dim3(TILE), 0, s, E);
@@ -413,7 +425,7 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   e->dev.stage = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * max_slots);
   e->dev.counters = reinterpret_cast<int64_t*>(p); p += al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2));
   e->dev.pre = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1));
-  e->dev.head = reinterpret_cast<int32_t*>(p);
+  e->dev.split = reinterpret_cast<int32_t*>(p);
   e->dev.n_slots = max_slots;
   for (int i = 0; i < max_slots; ++i) e->h_stage0[i] = -1;
   if (hipMemcpy(e->dev.stage, e->h_stage0, sizeof(int32_t) * max_slots, hipMemcpyHostToDevice) != hipSuccess) {
@@ -428,7 +440,6 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     cus = prop.multiProcessorCount;
   e->grid = cus * 4;
-  e->hit_grid = cus * 8;
   *out = e;
   return SPLAT_OK;
 }
@@ -438,7 +449,8 @@ int32_t splat_engine_destroy(splat_engine_t* e) {
   (void)hipDeviceSynchronize();
   for (auto& p : e->pending) (void)hipEventDestroy(p.ev);
   for (auto ev : e->free_events) (void)hipEventDestroy(ev);
-  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  for (auto& g : e->graphs)
+    if (g.second) (void)hipGraphExecDestroy(g.second);
   if (e->cap_stream) (void)hipStreamDestroy(e->cap_stream);
   if (e->h_slots) (void)hipHostFree(e->h_slots);
   if (e->h_lm) (void)hipHostFree(e->h_lm);
@@ -468,7 +480,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   if (slot < 0) return SPLAT_ERR_BUDGET;
   HostSlot& hsl = e->slots[slot];
   const int64_t tiles = (R + TILE - 1) / TILE;
-  std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 4, tiles * 4, tiles * 4};
+  std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 8, tiles * 4, tiles * 4};
   if (!rq->rays_out) sizes.push_back(R * (int64_t)sizeof(splat_adray_t));
   if (!rq->hit_out) sizes.push_back(R * 4);
   std::vector<Block> got;
@@ -485,7 +497,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   std::memset(&D, 0, sizeof(D));
   D.S[0] = reinterpret_cast<splat_adray_t*>(at(0));
   D.S[1] = reinterpret_cast<splat_adray_t*>(at(1));
-  D.hit_scratch = reinterpret_cast<int32_t*>(at(2));
+  D.keys = reinterpret_cast<uint64_t*>(at(2));
   D.tile_counts = reinterpret_cast<int32_t*>(at(3));
   D.tile_offsets = reinterpret_cast<int32_t*>(at(4));
   size_t k = 5;
@@ -517,6 +529,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   hsl.used = true;
   hsl.stage = 0;
   hsl.depth = rq->depth;
+  hsl.R = R;
   hsl.tag = rq->tag;
   hsl.blocks = got;
   return SPLAT_OK;
@@ -526,21 +539,36 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   if (!e) return SPLAT_ERR_INVALID_ARG;
   hipStream_t s = static_cast<hipStream_t>(stream);
   bool any = false;
-  for (auto& h : e->slots) any |= h.used && h.stage >= 0;
+  int64_t groups = 0;  // upper bound of the live 64-ray groups in this step
+  for (auto& h : e->slots) {
+    if (!h.used || h.stage < 0) continue;
+    any = true;
+    groups += (h.R + HIT_ITEM - 1) / HIT_ITEM;
+  }
   if (!any) return SPLAT_OK;
+  // hitter grid: a power of two of workgroups (4 waves each) that covers every item the live
+  // counts can need, so a few graphs serve any mix of frames
+  const int64_t need = (max_hit_items(groups) + 3) / 4;
+  int64_t hit_blocks = 1;
+  while (hit_blocks < need) hit_blocks <<= 1;
+  if (hit_blocks > (int64_t)1 << 24) return SPLAT_ERR_UNSUPPORTED;  // grid of at most 2^32 threads
   if (e->use_graph) {
-    if (!e->graph) {
+    hipGraphExec_t& ge = e->graphs[hit_blocks];
+    if (!ge) {
       hipGraph_t g;
       if (!e->cap_stream && hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess) return SPLAT_ERR_HIP;
       if (hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPLAT_ERR_HIP;
-      int32_t st = launch_step_kernels(e, e->cap_stream);
+      int32_t st = launch_step_kernels(e, hit_blocks, e->cap_stream);
       if (hipStreamEndCapture(e->cap_stream, &g) != hipSuccess || st != SPLAT_OK) return SPLAT_ERR_HIP;
-      hipError_t ie = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+      hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
-      if (ie != hipSuccess) return SPLAT_ERR_HIP;
+      if (ie != hipSuccess) {
+        ge = nullptr;
+        return SPLAT_ERR_HIP;
+      }
     }
-    if (hipGraphLaunch(e->graph, s) != hipSuccess) return SPLAT_ERR_HIP;
-  } else if (int32_t st = launch_step_kernels(e, s)) {
+    if (hipGraphLaunch(ge, s) != hipSuccess) return SPLAT_ERR_HIP;
+  } else if (int32_t st = launch_step_kernels(e, hit_blocks, s)) {
     return st;
   }
   Pending p;
@@ -599,6 +627,13 @@ int32_t splat_engine_in_flight(const splat_engine_t* e) {
   if (!e) return 0;
   int32_t n = 0;
   for (auto& h : e->slots) n += h.used ? 1 : 0;
+  return n;
+}
+
+int32_t splat_engine_runnable(const splat_engine_t* e) {
+  if (!e) return 0;
+  int32_t n = 0;
+  for (auto& h : e->slots) n += h.used && h.stage >= 0 ? 1 : 0;
   return n;
 }
 

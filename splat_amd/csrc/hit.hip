@@ -91,7 +91,8 @@ template <int W, bool STATS>
 __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restrict__ rays, int64_t R,
                                                    const int64_t* __restrict__ d_count,
                                                    const TriRec* __restrict__ recs, int32_t N,
-                                                   int32_t* __restrict__ hit_idx, unsigned long long* stats) {
+                                                   int32_t* __restrict__ hit_idx, uint64_t* __restrict__ keys,
+                                                   unsigned long long* stats) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t Rl = live_count(R, d_count);
@@ -121,47 +122,67 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
     }
   }
   if constexpr (W == 1) {
-    if (valid) hit_idx[r] = bi;
+    if (valid) {
+      if (keys) keys[r] = pack_key(best, bi);
+      else hit_idx[r] = bi;
+    }
   } else {
-    __shared__ uint64_t keys[W][64];
-    keys[wave][lane] = pack_key(best, bi);
+    __shared__ uint64_t part[W][64];
+    part[wave][lane] = pack_key(best, bi);
     __syncthreads();
     if (wave == 0) {
-      uint64_t k = keys[0][lane];
+      uint64_t k = part[0][lane];
 #pragma unroll
       for (int w = 1; w < W; ++w) {
-        uint64_t o = keys[w][lane];
+        uint64_t o = part[w][lane];
         k = o < k ? o : k;
       }
-      if (valid) hit_idx[r] = key_index(k);
+      if (valid) {
+        if (keys) keys[r] = k;
+        else hit_idx[r] = key_index(k);
+      }
     }
   }
 }
 
+// BV hitter over (64-ray group, split) work items, one wave each. The split factor follows the
+// live count on the device, so bounces with few, incoherent survivors still spread over the whole
+// chip; the splits of a group merge their (t, index) keys with a 64-bit atomic minimum. The grid
+// covers the most items any live count can need (max_hit_items); waves past the count leave.
 template <bool STATS, bool GROUPS>
 __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
-                                                const int64_t* __restrict__ d_count,
-                                                const float4* __restrict__ boxes, const int2* __restrict__ ranges,
-                                                const TriRec* __restrict__ recs, int32_t n_bv,
-                                                const float4* __restrict__ groups, int32_t gsize, int32_t n_groups,
-                                                int32_t* __restrict__ hit_idx, unsigned long long* stats) {
-  const int64_t Rl = live_count(R, d_count);
-  const int64_t wave_base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
-  if (wave_base >= Rl) return;  // uniform over the wave
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool valid = r < Rl;
-  const RayLanes L = load_ray_lanes(rays, r, valid);
+                                                const int64_t* __restrict__ d_count, BVScene scene,
+                                                uint64_t* __restrict__ keys, unsigned long long* stats) {
+  const int64_t A = live_count(R, d_count);
+  const int64_t groups64 = (A + 63) / 64;
+  const int32_t units = GROUPS ? scene.n_groups : scene.n_bv;
+  const int32_t S = split_factor(groups64, units);
+  const int64_t i = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (i >= groups64 * S) return;  // whole waves
   HitStats st;
-  const int32_t bi = bv_traverse<STATS, GROUPS>(L, valid, BVScene{boxes, ranges, recs, groups, n_bv, gsize, n_groups}, st);
-  if (valid) hit_idx[r] = bi;
+  const int64_t g = i / S;
+  bv_item<STATS, GROUPS>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st);
   if constexpr (STATS) {
-    if (valid) {
-      atomicAdd(stats + 0, st.n_box);
-      atomicAdd(stats + 1, st.n_enter);
-      atomicAdd(stats + 2, st.n_alg);
-    }
+    atomicAdd(stats + 0, st.n_box);
+    atomicAdd(stats + 1, st.n_enter);
+    atomicAdd(stats + 2, st.n_alg);
     if ((threadIdx.x & 63) == 0) atomicAdd(stats + 3, 64ull * st.n_exec);
   }
+}
+
+// keys of the live rays <- (Inf, 1)
+__global__ void k_key_init(uint64_t* __restrict__ keys, int64_t R, const int64_t* __restrict__ d_count) {
+  const int64_t A = live_count(R, d_count);
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A; r += (int64_t)gridDim.x * blockDim.x)
+    keys[r] = KEY_MISS;
+}
+
+// hit_idx[r] <- index part of keys[r] for the live rays (retreive_arg, src/atomic_argmin.jl:17-20)
+__global__ void k_key_final(const uint64_t* __restrict__ keys, int64_t R, const int64_t* __restrict__ d_count,
+                            int32_t* __restrict__ hit_idx) {
+  const int64_t A = live_count(R, d_count);
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A; r += (int64_t)gridDim.x * blockDim.x)
+    hit_idx[r] = key_index(keys[r]);
 }
 
 // group boxes: union of each group's (already padded) leaf boxes
@@ -207,23 +228,23 @@ int choose_bf_waves(int64_t R, int32_t N) {
 
 template <bool ST>
 static void launch_hit_bf_t(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
-                            int32_t* hit_idx, int waves, unsigned long long* st, hipStream_t s) {
+                            int32_t* hit_idx, uint64_t* keys, int waves, unsigned long long* st, hipStream_t s) {
   dim3 grid((unsigned)((R + 63) / 64));
   switch (waves) {
-    case 1: hipLaunchKernelGGL((k_hit_bf<1, ST>), grid, dim3(64), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
-    case 2: hipLaunchKernelGGL((k_hit_bf<2, ST>), grid, dim3(128), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
-    case 4: hipLaunchKernelGGL((k_hit_bf<4, ST>), grid, dim3(256), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
-    case 8: hipLaunchKernelGGL((k_hit_bf<8, ST>), grid, dim3(512), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
-    default: hipLaunchKernelGGL((k_hit_bf<16, ST>), grid, dim3(1024), 0, s, rays, R, d_count, recs, N, hit_idx, st); break;
+    case 1: hipLaunchKernelGGL((k_hit_bf<1, ST>), grid, dim3(64), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
+    case 2: hipLaunchKernelGGL((k_hit_bf<2, ST>), grid, dim3(128), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
+    case 4: hipLaunchKernelGGL((k_hit_bf<4, ST>), grid, dim3(256), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
+    case 8: hipLaunchKernelGGL((k_hit_bf<8, ST>), grid, dim3(512), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
+    default: hipLaunchKernelGGL((k_hit_bf<16, ST>), grid, dim3(1024), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
   }
 }
 
 void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
-                   int32_t* hit_idx, int waves, uint64_t* stats, hipStream_t s) {
+                   int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
-  if (st) launch_hit_bf_t<true>(rays, R, d_count, recs, N, hit_idx, waves, st, s);
-  else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, waves, st, s);
+  if (st) launch_hit_bf_t<true>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, s);
+  else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, s);
 }
 
 void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t gsize, float4* groups, hipStream_t s) {
@@ -231,21 +252,31 @@ void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t gsize, float4
   hipLaunchKernelGGL(k_group_boxes, dim3((n_groups + 63) / 64), dim3(64), 0, s, boxes, n_bv, gsize, groups);
 }
 
-void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const float4* boxes, const int2* ranges,
-                   const TriRec* recs, int32_t n_bv, const float4* groups, int32_t gsize, int32_t n_groups,
-                   int32_t* hit_idx, uint64_t* stats, hipStream_t s) {
+static unsigned stream_blocks(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (unsigned)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
+}
+
+void launch_key_init(uint64_t* keys, int64_t R, const int64_t* d_count, hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(k_key_init, dim3(stream_blocks(R)), dim3(256), 0, s, keys, R, d_count);
+}
+void launch_key_final(const uint64_t* keys, int64_t R, const int64_t* d_count, int32_t* hit_idx, hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(k_key_final, dim3(stream_blocks(R)), dim3(256), 0, s, keys, R, d_count, hit_idx);
+}
+
+// keys[] must have been initialised (launch_key_init) on the same stream
+void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const BVScene& scene, uint64_t* keys,
+                   uint64_t* stats, hipStream_t s) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
-  dim3 grid((unsigned)((R + 255) / 256)), block(256);
-  const bool g = groups != nullptr && gsize > 0 && n_groups > 0;
-  if (st && g)
-    hipLaunchKernelGGL((k_hit_bv<true, true>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
-  else if (st)
-    hipLaunchKernelGGL((k_hit_bv<true, false>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
-  else if (g)
-    hipLaunchKernelGGL((k_hit_bv<false, true>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
-  else
-    hipLaunchKernelGGL((k_hit_bv<false, false>), grid, block, 0, s, rays, R, d_count, boxes, ranges, recs, n_bv, groups, gsize, n_groups, hit_idx, st);
+  dim3 grid((unsigned)((max_hit_items((R + 63) / 64) + 3) / 4)), block(256);
+  const bool g = scene.groups != nullptr && scene.gsize > 0 && scene.n_groups > 0;
+  if (st && g) hipLaunchKernelGGL((k_hit_bv<true, true>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
+  else if (st) hipLaunchKernelGGL((k_hit_bv<true, false>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
+  else if (g) hipLaunchKernelGGL((k_hit_bv<false, true>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
+  else hipLaunchKernelGGL((k_hit_bv<false, false>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
 }
 
 }  // namespace spl

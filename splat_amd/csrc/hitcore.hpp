@@ -46,29 +46,45 @@ __device__ __forceinline__ bool slab_enter(const float4& lo, const float4& hi, c
   return (tmin <= tmax) && (tmin <= best);
 }
 
-// Prepared scene of the BV hitter as the kernels see it.
-struct BVScene {
-  const float4* boxes;
-  const int2* ranges;
-  const TriRec* recs;
-  const float4* groups;
-  int32_t n_bv, gsize, n_groups;
-};
-
 struct HitStats {
   unsigned long long n_box = 0, n_enter = 0, n_alg = 0, n_exec = 0;
 };
+
+// (t, index) key of a miss: (Inf, 1)
+constexpr uint64_t KEY_MISS = 0xFF80000000000001ull;
+
+__device__ __forceinline__ float key_t(uint64_t k) {
+  const uint32_t m = (uint32_t)(k >> 32);
+  return __uint_as_float((m >> 31) == 0 ? ~m : (m ^ 0x80000000u));
+}
 
 // One wave, one ray per lane: the lexicographic (t, index) minimum over the scene's BV leaves,
 // visited in order, optionally under the coarse level of groups of `gsize` consecutive leaves.
 // A box no lane enters (one ballot) is skipped with its members; otherwise all lanes test its
 // member records (a triangle held by two leaves ties with itself). Must be called by whole waves.
+// Scene data is read-only for the whole launch: reading it through the constant address space
+// lets every wave-uniform access become a scalar load, whatever the kernel writes elsewhere.
+// (The host pass only parses these functions; it has no such address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <class T>
+using cptr = const T*;
+#endif
+template <class T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)(p);
+}
+
 template <bool STATS, bool GROUPS>
-__device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, const BVScene& S, HitStats& st) {
-  const float4* __restrict__ boxes = S.boxes;
-  const int2* __restrict__ ranges = S.ranges;
-  const TriRec* __restrict__ recs = S.recs;
-  const float4* __restrict__ groups = S.groups;
+__device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, const BVScene& S, HitStats& st,
+                                               float* best_io = nullptr, int32_t split = 0, int32_t n_split = 1,
+                                               const uint64_t* shared_key = nullptr) {
+  const cptr<float4> boxes = as_const(S.boxes);
+  const cptr<int2> ranges = as_const(S.ranges);
+  const cptr<TriRec> recs = as_const(S.recs);
+  const cptr<float4> groups = as_const(S.groups);
   const int32_t n_bv = S.n_bv, gsize = S.gsize, n_groups = S.n_groups;
   unsigned long long& n_box = st.n_box;
   unsigned long long& n_enter = st.n_enter;
@@ -81,9 +97,21 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
   const float ix = safe_inv(L.dx), iy = safe_inv(L.dy), iz = safe_inv(L.dz);
   float best = INFINITY;
   int32_t bi = 1;
+  // another split of the same rays may already hold a nearer hit: prune with it (it is a real
+  // hit of this ray, so the lexicographic minimum is unchanged)
+  auto refresh = [&]() {
+    if (shared_key && valid) {
+      const uint64_t k = __hip_atomic_load(shared_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k < pack_key(best, bi)) {
+        best = key_t(k);
+        bi = key_index(k);
+      }
+    }
+  };
 
   auto scan_leaf = [&](int32_t b) {
-    const bool enter = valid && slab_enter(boxes[2 * b], boxes[2 * b + 1], L, ix, iy, iz, best);
+    const float4 blo = boxes[2 * b], bhi = boxes[2 * b + 1];
+    const bool enter = valid && slab_enter(blo, bhi, L, ix, iy, iz, best);
     if constexpr (STATS) n_box += valid ? 1ull : 0ull;
     if (__builtin_amdgcn_ballot_w64(enter) == 0) return;
     const int2 rg = ranges[b];
@@ -94,10 +122,10 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     }
 #pragma unroll 4
     for (int32_t m = 0; m < rg.y; ++m) {
-      const TriRec& T = recs[rg.x + m];
+      const float4 q0 = recs[rg.x + m].q0, q1 = recs[rg.x + m].q1, q2 = recs[rg.x + m].q2, q3 = recs[rg.x + m].q3;
       float t;
-      bool h = hit_rec(T.q0, T.q1, T.q2, T.q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
-      int32_t idx = __float_as_int(T.q1.w);
+      bool h = hit_rec(q0, q1, q2, q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
+      int32_t idx = __float_as_int(q1.w);
       if (h && (t < best || (t == best && idx < bi))) {
         best = t;
         bi = idx;
@@ -106,17 +134,58 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
   };
 
   if constexpr (GROUPS) {
-    for (int32_t g = 0; g < n_groups; ++g) {
-      const bool enter = valid && slab_enter(groups[2 * g], groups[2 * g + 1], L, ix, iy, iz, best);
+    for (int32_t g = split; g < n_groups; g += n_split) {
+      refresh();
+      const float4 glo = groups[2 * g], ghi = groups[2 * g + 1];
+      const bool enter = valid && slab_enter(glo, ghi, L, ix, iy, iz, best);
       if constexpr (STATS) n_box += valid ? 1ull : 0ull;
       if (__builtin_amdgcn_ballot_w64(enter) == 0) continue;
       const int32_t b1 = (g + 1) * gsize < n_bv ? (g + 1) * gsize : n_bv;
       for (int32_t b = g * gsize; b < b1; ++b) scan_leaf(b);
     }
   } else {
-    for (int32_t b = 0; b < n_bv; ++b) scan_leaf(b);
+    for (int32_t b = split; b < n_bv; b += n_split) {
+      if (((b - split) & 15) == 0) refresh();
+      scan_leaf(b);
+    }
   }
+  if (best_io) *best_io = best;
   return bi;
+}
+
+// Split factor for `groups64` live 64-ray groups over a scene with `units` top-level boxes:
+// enough (group, split) items to fill the chip, never more splits than boxes.
+constexpr int64_t SPLIT_TARGET_ITEMS = 16384;
+constexpr int32_t MAX_SPLIT = 32;
+__device__ __forceinline__ int32_t split_factor(int64_t groups64, int32_t units) {
+  int32_t S = 1;
+  while (S < MAX_SPLIT && S < units && groups64 * S < SPLIT_TARGET_ITEMS) S *= 2;
+  return S;
+}
+// Work items (waves) the hitter needs for at most `groups64` live groups: the split factor only
+// grows while groups64 * S < SPLIT_TARGET_ITEMS, so it never takes the count past twice that.
+inline __host__ __device__ int64_t max_hit_items(int64_t groups64) {
+  return groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS;
+}
+
+// One (64-ray group, split) work item: rays [base, base + 64) of `rays` with A live, scanning
+// split `split` of `n_split`; the result is merged into keys[r] with a 64-bit atomic minimum of
+// the packed (t, index) key. Called by whole waves.
+template <bool STATS, bool GROUPS>
+__device__ __forceinline__ void bv_item(const splat_adray_t* __restrict__ rays, int64_t A, int64_t base, int32_t split,
+                                        int32_t n_split, const BVScene& scene, uint64_t* __restrict__ keys,
+                                        HitStats& st) {
+  const int64_t r = base + (threadIdx.x & 63);
+  const bool valid = r < A;
+  const RayLanes L = load_ray_lanes(rays, r, valid);
+  float best;
+  const int32_t bi = bv_traverse<STATS, GROUPS>(L, valid, scene, st, &best, split, n_split,
+                                                n_split > 1 && valid ? keys + r : nullptr);
+  if (valid) {
+    const uint64_t k = pack_key(best, bi);
+    if (n_split == 1) keys[r] = k;
+    else if (k < KEY_MISS) atomicMin(reinterpret_cast<unsigned long long*>(keys + r), (unsigned long long)k);
+  }
 }
 
 }  // namespace spl

@@ -14,15 +14,28 @@ void launch_prep_members(const splat_ftri_t* tris, const int32_t* members, int64
 void launch_bv_boxes(const splat_ftri_t* tris, const int32_t* members, const int2* ranges, int32_t n_bv, float4* boxes,
                      hipStream_t s);
 int choose_bf_waves(int64_t R, int32_t N);
+// brute force: writes hit_idx, or the packed (t, index) keys when keys != nullptr
 void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
-                   int32_t* hit_idx, int waves, uint64_t* stats, hipStream_t s);
+                   int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s);
 void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, float4* groups, hipStream_t s);
-void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const float4* boxes, const int2* ranges,
-                   const TriRec* recs, int32_t n_bv, const float4* groups, int32_t group_size, int32_t n_groups,
-                   int32_t* hit_idx, uint64_t* stats, hipStream_t s);
+
+// Prepared scene of the BV hitter as the kernels see it.
+struct BVScene {
+  const float4* boxes;
+  const int2* ranges;
+  const TriRec* recs;
+  const float4* groups;
+  int32_t n_bv, gsize, n_groups;
+};
+// BV hitter into packed keys (initialised by launch_key_init on the same stream)
+void launch_key_init(uint64_t* keys, int64_t R, const int64_t* d_count, hipStream_t s);
+void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const BVScene& scene, uint64_t* keys,
+                   uint64_t* stats, hipStream_t s);
+void launch_key_final(const uint64_t* keys, int64_t R, const int64_t* d_count, int32_t* hit_idx, hipStream_t s);
 
 // trace.hip
-void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
+// hit index per ray from hit_idx, or from the packed keys when keys != nullptr
+void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const uint64_t* keys,
                    const splat_ftri_t* tris, int32_t N, const float* rndm, float force_rand, uint64_t seed, int32_t bounce,
                    int32_t first_diffuse, const splat_optics_t& opt, int32_t* tile_counts, hipStream_t s);
 void launch_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
@@ -43,6 +56,6 @@ void launch_scan_tiles(const int32_t* tile_counts, int64_t n_tiles, const int64_
 // the optional carry array moves with the rest part (carry_rest aligned with out_rest).
 void launch_scatter(const splat_adray_t* in, int64_t R, const int64_t* d_count, const int32_t* tile_offsets,
                     const int64_t* d_total, splat_adray_t* out_active, splat_adray_t* out_rest, const int32_t* carry_in,
-                    int32_t* carry_rest, hipStream_t s);
+                    const uint64_t* carry_keys, int32_t* carry_rest, hipStream_t s);
 
 }  // namespace spl

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(TILE) void k_scatter(const splat_adray_t* __restric
                                                   splat_adray_t* __restrict__ out_active,
                                                   splat_adray_t* __restrict__ out_rest,
                                                   const int32_t* __restrict__ carry_in,
+                                                  const uint64_t* __restrict__ carry_keys,
                                                   int32_t* __restrict__ carry_rest) {
   const int64_t Rl = live_count_p(R, d_count);
   const int64_t tile_base = (int64_t)blockIdx.x * TILE;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(TILE) void k_scatter(const splat_adray_t* __restric
   const int64_t off_a = tile_offsets[blockIdx.x];
   const int64_t off_r = *d_total + (tile_base - off_a);  // non-ACTIVE rays before this tile
   scatter_tile(in + tile_base, nvalid, off_a, off_r, out_active, out_rest, carry_in ? carry_in + tile_base : nullptr,
-               carry_rest);
+               carry_rest, carry_keys ? carry_keys + tile_base : nullptr);
 }
 
 void launch_count_active(const splat_adray_t* rays, int64_t R, const int64_t* d_count, int32_t* tile_counts,
@@ -84,10 +85,10 @@ void launch_scan_tiles(const int32_t* tile_counts, int64_t n_tiles, const int64_
 }
 void launch_scatter(const splat_adray_t* in, int64_t R, const int64_t* d_count, const int32_t* tile_offsets,
                     const int64_t* d_total, splat_adray_t* out_active, splat_adray_t* out_rest, const int32_t* carry_in,
-                    int32_t* carry_rest, hipStream_t s) {
+                    const uint64_t* carry_keys, int32_t* carry_rest, hipStream_t s) {
   if (R <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, in, R, d_count, tile_offsets,
-                     d_total, out_active, out_rest, carry_in, carry_rest);
+                     d_total, out_active, out_rest, carry_in, carry_keys, carry_rest);
 }
 
 }  // namespace spl

@@ -8,12 +8,14 @@ namespace spl {
 constexpr int WORDS = sizeof(splat_adray_t) / 8;  // 17 eight-byte words per record
 
 // Stable scatter of one tile of `nvalid` rays starting at `in`: ACTIVE rays to out_active[off_a..],
-// the others to out_rest[off_r..], both in input order; the carry array moves with the others.
+// the others to out_rest[off_r..], both in input order; the carry array (hit indices, or the index
+// parts of packed keys) moves with the others.
 // Called by a whole TILE-thread workgroup.
 __device__ __forceinline__ void scatter_tile(const splat_adray_t* __restrict__ in, int nvalid, int64_t off_a,
                                              int64_t off_r, splat_adray_t* __restrict__ out_active,
                                              splat_adray_t* __restrict__ out_rest, const int32_t* __restrict__ carry_in,
-                                             int32_t* __restrict__ carry_rest) {
+                                             int32_t* __restrict__ carry_rest,
+                                             const uint64_t* __restrict__ carry_keys = nullptr) {
   __shared__ uint2 stage[TILE * WORDS];  // 34,816 bytes
   __shared__ int16_t order[TILE];
   __shared__ int32_t wave_a[TILE / 64], wave_r[TILE / 64];
@@ -42,7 +44,7 @@ __device__ __forceinline__ void scatter_tile(const splat_adray_t* __restrict__ i
   }
   if (act) order[pa + ra] = (int16_t)tid;
   if (rest) order[na + pr + rr] = (int16_t)tid;
-  if (rest && carry_rest) carry_rest[off_r + pr + rr] = carry_in[tid];
+  if (rest && carry_rest) carry_rest[off_r + pr + rr] = carry_keys ? key_index(carry_keys[tid]) : carry_in[tid];
   __syncthreads();
 
   // coalesced writes of the two output runs

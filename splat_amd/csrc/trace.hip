@@ -6,6 +6,7 @@ namespace spl {
 __global__ __launch_bounds__(TILE) void k_evolve(splat_adray_t* __restrict__ rays, int64_t R,
                                                  const int64_t* __restrict__ d_count,
                                                  const int32_t* __restrict__ hit_idx,
+                                                 const uint64_t* __restrict__ keys,
                                                  const splat_ftri_t* __restrict__ tris, int32_t N,
                                                  const float* __restrict__ rndm, float force_rand, uint64_t seed,
                                                  int32_t bounce, int32_t first_diffuse, splat_optics_t opt,
@@ -15,7 +16,7 @@ __global__ __launch_bounds__(TILE) void k_evolve(splat_adray_t* __restrict__ ray
   bool active_after = false;
   if (r < Rl && rays[r].status == SPLAT_RAY_ACTIVE) {
     RayF ray = load_rayf(rays + r);
-    int32_t i = hit_idx[r];
+    int32_t i = keys ? key_index(keys[r]) : hit_idx[r];
     if (i < 1 || i > N) i = 1;  // an index outside tris reads as the miss sentinel
     FTriF F = load_ftri(tris + (i - 1));
     float u = bounce_random(force_rand, rndm, r, seed, bounce, ray.dest);
@@ -64,11 +65,12 @@ __global__ void k_copy_rays(const uint2* __restrict__ src, uint2* __restrict__ d
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
-void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const splat_ftri_t* tris,
+void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const uint64_t* keys,
+                   const splat_ftri_t* tris,
                    int32_t N, const float* rndm, float force_rand, uint64_t seed, int32_t bounce, int32_t first_diffuse,
                    const splat_optics_t& opt, int32_t* tile_counts, hipStream_t s) {
   if (R <= 0) return;
-  hipLaunchKernelGGL(k_evolve, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, hit_idx, tris, N,
+  hipLaunchKernelGGL(k_evolve, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, hit_idx, keys, tris, N,
                      rndm, force_rand, seed, bounce, first_diffuse, opt, tile_counts);
 }
 void launch_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const splat_ftri_t* tris,

@@ -49,7 +49,8 @@ SIGNATURES = {
     "splat_hitset_prepare_bf_ftri": (i32, [P, i32, P, sz, P, P]),
     "splat_hitset_bv_bytes": (sz, [i32, i64]),
     "splat_hitset_prepare_bv": (i32, [P, i32, i32, P, P, i64, i32, P, sz, P, P]),
-    "splat_next_hit": (i32, [P, i64, P, P, P, P]),
+    "splat_next_hit_ws_bytes": (sz, [i64]),
+    "splat_next_hit": (i32, [P, i64, P, P, P, P, sz, P]),
     "splat_next_hit_bf": (i32, [P, i64, P, i32, P, P, sz, P]),
     "splat_evolve": (i32, [P, i64, P, P, P, i32, P, f32, u64, i32, i32, P, P]),
     "splat_partition_ws_bytes": (sz, [i64]),
@@ -68,6 +69,7 @@ SIGNATURES = {
     "splat_engine_step": (i32, [P, P]),
     "splat_engine_poll": (i32, [P, i32, P, i32, P]),
     "splat_engine_in_flight": (i32, [P]),
+    "splat_engine_runnable": (i32, [P]),
     "splat_engine_sync_steps": (i32, [P, i32]),
     "splat_engine_arena_free": (i64, [P]),
     "splat_engine_use_graph": (i32, [P, i32]),

@@ -99,15 +99,14 @@ def run_requests(engine: api.Engine, scenes: list[ResidentScene], requests: list
             rq.admitted = t
             queue.pop(0)
         max_in_flight = max(max_in_flight, engine.in_flight)
-        if engine.in_flight:
+        if engine.runnable:
             engine.step()
             steps += 1
             engine.sync_steps(max_pending_steps)
-        elif queue:  # idle until the next arrival
+        elif queue and not engine.in_flight:  # idle until the next arrival
             time.sleep(max(0.0, min(queue[0].arrival - now(), 0.002)))
-        tags = engine.poll(wait=False)
-        if not tags and engine.in_flight and not queue:
-            tags = engine.poll(wait=True)
+        # with nothing left to launch, block on the completions still in flight
+        tags = engine.poll(wait=engine.in_flight > 0 and engine.runnable == 0)
         tn = now()
         for tg in tags:
             by_tag[tg].done = tn

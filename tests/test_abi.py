@@ -63,10 +63,11 @@ def test_workspace_sizes_are_monotone():
 def test_argument_errors_without_a_gpu():
     """Invalid arguments are rejected before anything is launched."""
     lib = L.lib()
-    assert lib.splat_next_hit(None, -1, None, None, None, None) == -1
+    assert lib.splat_next_hit(None, -1, None, None, None, None, 0, None) == -1
     assert lib.splat_partition_active(None, None, -5, None, None, None, 0, None) == -1
     hs = abi.HitSet()
-    assert lib.splat_next_hit(None, 0, None, L.ref(hs), None, None) == -1  # no records
+    assert lib.splat_next_hit(None, 0, None, L.ref(hs), None, None, 0, None) == -1  # no records
+    assert lib.splat_next_hit_ws_bytes(1000) >= 8 * 1000 and lib.splat_next_hit_ws_bytes(-1) == 0
     p = abi.run_params(65, 2)
     assert lib.splat_run_evolution(C.c_void_p(8), 10, L.ref(hs), C.c_void_p(8), 3, C.c_void_p(8), L.ref(p),
                                    C.c_void_p(8), 1 << 30, None, None) == -1

@@ -114,8 +114,9 @@ def test_engine_continuous_admission_reuses_slots(scenes):
     for _ in range(200):
         if queue and _admit(eng, frames[queue[0]], queue[0]):
             queue.pop(0)
-        eng.step()
-        done += eng.poll(wait=False)
+        if eng.runnable:
+            eng.step()
+        done += eng.poll(wait=eng.runnable == 0)  # nothing left to launch: wait for completions
         if not queue and eng.in_flight == 0:
             break
     done += eng.poll(wait=True)
