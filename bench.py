@@ -3,7 +3,8 @@
 
 Workload: two displaced icospheres (subdivision 4 each, 10,240 glass triangles, radial noise
 sigma 0.02) over a diffuse plane; 1024 x 1024 rays from a pinhole at the point light, one seeded
-wavelength per ray, Cauchy glass (A 1.5, B 0.004 um^2), one-level bounding-volume hitter, 4
+wavelength per ray, Cauchy glass (A 1.5, B 0.004 um^2), bounding-volume hitter (tree over 2^10
+object-median leaves), 4
 bounces with dual-number differentials, splat into a 1024 x 1024 RGB float32 light map.
 A step is one whole frame: clear map -> pinhole rays -> bounce loop (hit, evolve, stable
 compaction) -> realigning hit -> final evolution + splat, all device-resident, one frame per
@@ -29,7 +30,6 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_TRI_TEST = 40  # distance_to_plane + in_triangle + p = o + t d, per-triangle terms hoisted
 FLOP_PER_BOX_TEST = 12  # slab test with precomputed reciprocals
-BV_DEPTH = 8
 
 
 class Hip:
@@ -66,7 +66,7 @@ def build_workload(device="cuda"):
 
     sc = scene.scene_c2(seed=2)
     cam = scene.camera_c2(seed=2, jitter=0.025)
-    bvs, mem = bv.object_partition(sc.ftris, BV_DEPTH)
+    bvs, mem = bv.object_partition(sc.ftris, bv.tree_depth(sc.n_tris))
     tris_d = api.to_device(sc.ftris, device)
     hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=0)
     params = abi.run_params(4, sc.first_diffuse, math.nan, seed=2024)
@@ -248,7 +248,7 @@ def main() -> int:
         "data": "synthetic: seeded procedural scene and pinhole rays (no dataset)",
         "config": {
             "workload": "1 MP light-map frame: 2 displaced icospheres (subdiv 4, 10,240 glass tris) + diffuse plane, "
-                        "BV hitter (2^8 object-median leaves under a 16-group coarse level), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
+                        "BV hitter (binary tree over 2^10 object-median leaves), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
                         "light map; plain path (one frame per launch sequence)",
             "rays_per_frame": R,
             "triangles": sc.n_tris,

@@ -121,11 +121,12 @@ typedef struct splat_hitset {
   void* d_recs;          /* device: prepared triangle records, 64 B each                 */
   void* d_boxes;         /* device: conservative member boxes, 32 B each (BV only)       */
   void* d_ranges;        /* device: int32 (first record, count) per BV (BV only)          */
-  void* d_groups;        /* device: boxes of groups of consecutive BVs, 32 B each (BV only) */
+  void* d_groups;        /* device: top-structure boxes, 32 B each (BV only): group boxes,  */
+                         /* or the binary tree's node boxes when tree_levels > 0             */
   int32_t group_size;    /* BVs per group; 0 = no coarse level (BV only)                  */
   int32_t n_groups;      /* ceil(n_bv / group_size)                                        */
   int32_t tune_waves;    /* triangle-split waves per workgroup (0 = autotuned default)   */
-  int32_t _pad;
+  int32_t tree_levels;   /* 1 + levels of the tree over the BVs; 0 = no tree (BV only)    */
   /* Optional device uint64[4] counters, accumulated by every hitter launch when non-NULL:
    * [0] ray-box slab tests, [1] (ray, box) pairs entered, [2] ray-triangle tests of the
    * algorithm (brute force: every live ray x every triangle; BV: the members of the boxes each
@@ -220,10 +221,12 @@ int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t n_tris_le
 /* Bounding-volume preparation from the reference's BV memberships (DPBVHitter constructor,
  * src/alg_types.jl:81-90, pack_bv_tris src/bv.jl:126-136). `bv_tri_count` (n_bv) and
  * `bv_tris` (Julia column-major n_bv x bv_stride matrix of 1-based indices) are HOST arrays.
- * The boxes are rebuilt from the members and padded so the test is conservative. A coarse
- * level groups `group_size` consecutive BVs under one box (0 = choose ~sqrt(n_bv), negative =
- * none: the reference's one-level structure). Any memberships that cover the triangles give
- * the brute-force result. */
+ * The boxes are rebuilt from the members and padded so the test is conservative. Over them:
+ * group_size == 0 -- a binary tree over the BVs in their order (children of node n: 2n, 2n+1),
+ *                    walked depth-first, nearer child first, by each wavefront (default);
+ * group_size > 0  -- one coarse level of boxes over `group_size` consecutive BVs;
+ * group_size < 0  -- nothing: every BV box is tested (the reference's one-level structure).
+ * Any memberships that cover the triangles give the brute-force result. */
 size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members);
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t n_tris_len, int32_t n_bv,
                                 const int64_t* bv_tri_count, const int64_t* bv_tris,

@@ -34,7 +34,7 @@ mutable struct HitSet
     kind::Int32; n_tris::Int32; n_bv::Int32; max_members::Int32
     total_members::Int64
     d_recs::Ptr{Cvoid}; d_boxes::Ptr{Cvoid}; d_ranges::Ptr{Cvoid}; d_groups::Ptr{Cvoid}
-    group_size::Int32; n_groups::Int32; tune_waves::Int32; _pad::Int32
+    group_size::Int32; n_groups::Int32; tune_waves::Int32; tree_levels::Int32
     d_stats::Ptr{UInt64}
     HitSet() = new(0, 0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, C_NULL)
 end

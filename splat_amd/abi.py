@@ -63,7 +63,7 @@ class HitSet(C.Structure):
         ("kind", C.c_int32), ("n_tris", C.c_int32), ("n_bv", C.c_int32), ("max_members", C.c_int32),
         ("total_members", C.c_int64), ("d_recs", C.c_void_p), ("d_boxes", C.c_void_p),
         ("d_ranges", C.c_void_p), ("d_groups", C.c_void_p), ("group_size", C.c_int32), ("n_groups", C.c_int32),
-        ("tune_waves", C.c_int32), ("_pad", C.c_int32), ("d_stats", C.c_void_p),
+        ("tune_waves", C.c_int32), ("tree_levels", C.c_int32), ("d_stats", C.c_void_p),
     ]
 
 

@@ -6,8 +6,8 @@ the fastest is kept, and choices are cached in memory and, optionally, in a JSON
 
 Tuned knobs:
   brute force -- waves per workgroup W that split the triangle list (1, 2, 4, 8, 16);
-  BV          -- leaf count of the object-median build (2^d) and the coarse group size
-                 (none, ~sqrt(leaves), or a fixed size).
+  BV          -- leaf count of the object-median build (2^d) and the top structure over the
+                 leaves (binary tree, none, or a coarse level of fixed-size groups).
 All candidates return identical hit indices (the tuner asserts it on the tuning rays).
 """
 from __future__ import annotations
@@ -80,15 +80,15 @@ def tune_bf(n_tris_dev, rays, cache_path: str | None = None) -> dict:
 
 
 def tune_bv(tris_dev, host_tris: np.ndarray, rays, cache_path: str | None = None,
-            depths: tuple[int, ...] | None = None, group_sizes: tuple[int, ...] = (-1, 0, 8, 32)) -> dict:
+            depths: tuple[int, ...] | None = None, group_sizes: tuple[int, ...] = (0, -1, 16)) -> dict:
     """Best (leaf depth, group size) for the BV hitter on these rays; returns a dict with
     "depth", "group_size", "ms" and builds the winning hitter under "hitter" (not cached)."""
     _load(cache_path)
     N = int(host_tris.shape[0])
     key = _key("bv", int(rays.shape[0]), N)
     if depths is None:
-        d0 = max(2, int(round(math.log2(max(N, 2) / 40))))
-        depths = tuple(d for d in (d0 - 1, d0, d0 + 1) if d >= 1)
+        d0 = bv.tree_depth(N)
+        depths = tuple(d for d in (d0 - 2, d0 - 1, d0, d0 + 1) if d >= 1)
     if key not in _cache:
         best, ref = None, None
         for d in depths:

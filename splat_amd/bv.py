@@ -11,6 +11,8 @@ triangles.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from . import abi
@@ -43,6 +45,13 @@ def bv_partition(ftris: np.ndarray, depth: int = 6):
         bvs[k]["max"] = v.max(axis=0)
         members.append(np.sort(p) + 1)
     return bvs, members
+
+
+def tree_depth(n_tris: int, per_leaf: float = 8.0) -> int:
+    """Leaf depth of the object-median build for about `per_leaf` triangles per leaf: the size
+    at which the tree hitter's node tests and member tests balanced on the 10k- and 178k-triangle
+    scenes (2^10 and 2^14 leaves)."""
+    return max(1, min(20, int(round(math.log2(max(n_tris, 2) / per_leaf)))))
 
 
 def object_partition(ftris: np.ndarray, depth: int = 8):

@@ -84,19 +84,6 @@ int32_t check_hitset(const splat_hitset_t* hs) {
   return SPLAT_OK;
 }
 
-BVScene bv_scene(const splat_hitset_t* hs) {
-  BVScene b;
-  b.boxes = static_cast<const float4*>(hs->d_boxes);
-  b.ranges = static_cast<const int2*>(hs->d_ranges);
-  b.recs = static_cast<const TriRec*>(hs->d_recs);
-  b.n_bv = hs->n_bv;
-  const bool g = hs->d_groups && hs->group_size > 0 && hs->n_groups > 0;
-  b.groups = g ? static_cast<const float4*>(hs->d_groups) : nullptr;
-  b.gsize = g ? hs->group_size : 0;
-  b.n_groups = g ? hs->n_groups : 0;
-  return b;
-}
-
 int bf_waves(const splat_hitset_t* hs, int64_t R) {
   return hs->tune_waves > 0 ? hs->tune_waves : choose_bf_waves(R, hs->n_tris);
 }
@@ -283,16 +270,31 @@ int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t N, void* 
   return launch_status();
 }
 
+// levels of the binary tree over n_bv leaves (2^levels >= n_bv)
+static int32_t tree_levels_for(int32_t n_bv) {
+  int32_t l = 0;
+  while ((1 << l) < n_bv) ++l;
+  return l;
+}
+
+// the top structure's boxes: group boxes (at most n_bv groups) or tree nodes (2^(levels+1))
+static size_t top_boxes(int32_t n_bv) {
+  size_t tree_nodes = (size_t)2 << tree_levels_for(n_bv);
+  return tree_nodes > (size_t)n_bv ? tree_nodes : (size_t)n_bv;
+}
+
 size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members) {
-  if (n_bv < 1 || total_members < 0) return 0;
-  return align_up((size_t)total_members * sizeof(TriRec)) + 2 * align_up((size_t)n_bv * 2 * sizeof(float4)) +
-         align_up((size_t)n_bv * sizeof(int2)) + align_up((size_t)(total_members > 0 ? total_members : 1) * sizeof(int32_t));
+  if (n_bv < 1 || total_members < 0 || tree_levels_for(n_bv) > MAX_TREE_LEVELS) return 0;
+  return align_up((size_t)total_members * sizeof(TriRec)) + align_up((size_t)n_bv * 2 * sizeof(float4)) +
+         align_up(top_boxes(n_bv) * 2 * sizeof(float4)) + align_up((size_t)n_bv * sizeof(int2)) +
+         align_up((size_t)(total_members > 0 ? total_members : 1) * sizeof(int32_t));
 }
 
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_bv, const int64_t* bv_tri_count,
                                 const int64_t* bv_tris, int64_t bv_stride, int32_t group_size, void* ws, size_t ws_bytes,
                                 splat_hitset_t* out, void* stream) {
   if (!tris || N < 1 || n_bv < 1 || !bv_tri_count || !bv_tris || bv_stride < 0 || !ws || !out) return SPLAT_ERR_INVALID_ARG;
+  if (tree_levels_for(n_bv) > MAX_TREE_LEVELS) return SPLAT_ERR_UNSUPPORTED;
   // flatten the Julia column-major (n_bv x stride) membership matrix; drop the sentinel index 1
   std::vector<int32_t> members;
   std::vector<int2> ranges((size_t)n_bv);
@@ -313,16 +315,14 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   }
   const int64_t total = (int64_t)members.size();
   if (ws_bytes < splat_hitset_bv_bytes(n_bv, total)) return SPLAT_ERR_WORKSPACE;
-  if (group_size == 0) {  // ~sqrt(n_bv) BVs per group balances the two levels' slab tests
-    group_size = 1;
-    while ((int64_t)group_size * group_size < n_bv) ++group_size;
-  }
+  const bool tree = group_size == 0;
+  const int32_t levels = tree_levels_for(n_bv);
   if (group_size < 0 || n_bv < 4) group_size = 0;
   const int32_t n_groups = group_size > 0 ? (n_bv + group_size - 1) / group_size : 0;
   Carver c(ws);
   TriRec* recs = c.take<TriRec>((size_t)total);
   float4* boxes = c.take<float4>((size_t)n_bv * 2);
-  float4* groups = c.take<float4>((size_t)n_bv * 2);
+  float4* groups = c.take<float4>(top_boxes(n_bv) * 2);
   int2* d_ranges = c.take<int2>((size_t)n_bv);
   int32_t* d_members = c.take<int32_t>((size_t)(total > 0 ? total : 1));
   hipStream_t s = S(stream);
@@ -332,7 +332,8 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
     return SPLAT_ERR_HIP;
   launch_prep_members(tris, d_members, total, recs, s);
   launch_bv_boxes(tris, d_members, d_ranges, n_bv, boxes, s);
-  if (n_groups > 0) launch_group_boxes(boxes, n_bv, group_size, groups, s);
+  if (tree) launch_tree_build(boxes, d_ranges, n_bv, levels, groups, s);
+  else if (n_groups > 0) launch_group_boxes(boxes, n_bv, group_size, groups, s);
   if (int32_t e = launch_status()) return e;
   // the host vectors must outlive the copies
   if (hipStreamSynchronize(s) != hipSuccess) return SPLAT_ERR_HIP;
@@ -345,9 +346,10 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   out->d_recs = recs;
   out->d_boxes = boxes;
   out->d_ranges = d_ranges;
-  out->d_groups = n_groups > 0 ? groups : nullptr;
-  out->group_size = group_size;
-  out->n_groups = n_groups;
+  out->d_groups = tree || n_groups > 0 ? groups : nullptr;
+  out->group_size = tree ? 0 : group_size;
+  out->n_groups = tree ? 0 : n_groups;
+  out->tree_levels = tree ? levels + 1 : 0;
   return SPLAT_OK;
 }
 

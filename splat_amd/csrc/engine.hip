@@ -160,8 +160,11 @@ __global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
     const int32_t st = E.stage[s];
     const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
     const int64_t base = (int64_t)(item - P[s]) * HIT_ITEM;
-    if (D.scene.groups) bv_item<false, true>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs);
-    else bv_item<false, false>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs);
+    switch (bv_mode(D.scene)) {
+      case BV_TREE: bv_item<false, BV_TREE>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs); break;
+      case BV_GROUPS: bv_item<false, BV_GROUPS>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs); break;
+      default: bv_item<false, BV_FLAT>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs); break;
+    }
   }
 }
 
@@ -505,10 +508,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   D.hit_final = rq->hit_out ? rq->hit_out : reinterpret_cast<int32_t*>(at(k++));
   D.tex = rq->tex;
   D.tris = rq->tris;
-  D.scene = BVScene{static_cast<const float4*>(rq->hs.d_boxes), static_cast<const int2*>(rq->hs.d_ranges),
-                    static_cast<const TriRec*>(rq->hs.d_recs),
-                    rq->hs.d_groups && rq->hs.group_size > 0 ? static_cast<const float4*>(rq->hs.d_groups) : nullptr,
-                    rq->hs.n_bv, rq->hs.group_size, rq->hs.n_groups};
+  D.scene = bv_scene(&rq->hs);
   D.R = R;
   D.tex_n = (int64_t)lm.width * lm.height * lm.channels;
   D.n_tris = rq->n_tris;

@@ -149,19 +149,19 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
 // live count on the device, so bounces with few, incoherent survivors still spread over the whole
 // chip; the splits of a group merge their (t, index) keys with a 64-bit atomic minimum. The grid
 // covers the most items any live count can need (max_hit_items); waves past the count leave.
-template <bool STATS, bool GROUPS>
+template <bool STATS, int MODE>
 __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
                                                 const int64_t* __restrict__ d_count, BVScene scene,
                                                 uint64_t* __restrict__ keys, unsigned long long* stats) {
   const int64_t A = live_count(R, d_count);
   const int64_t groups64 = (A + 63) / 64;
-  const int32_t units = GROUPS ? scene.n_groups : scene.n_bv;
+  const int32_t units = MODE == BV_TREE ? (1 << scene.levels) : MODE == BV_GROUPS ? scene.n_groups : scene.n_bv;
   const int32_t S = split_factor(groups64, units);
   const int64_t i = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (i >= groups64 * S) return;  // whole waves
   HitStats st;
   const int64_t g = i / S;
-  bv_item<STATS, GROUPS>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st);
+  bv_item<STATS, MODE>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st);
   if constexpr (STATS) {
     atomicAdd(stats + 0, st.n_box);
     atomicAdd(stats + 1, st.n_enter);
@@ -199,6 +199,57 @@ __global__ void k_group_boxes(const float4* __restrict__ boxes, int32_t n_bv, in
   }
   groups[2 * g] = lo;
   groups[2 * g + 1] = hi;
+}
+
+// tree leaves: node 2^L + b <- box of BV b (empty when b is padding or has no members)
+__global__ void k_tree_leaves(const float4* __restrict__ boxes, const int2* __restrict__ ranges, int32_t n_bv,
+                              int32_t levels, float4* __restrict__ nodes) {
+  const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t n_leaves = 1 << levels;
+  if (b >= n_leaves) return;
+  const int32_t n = n_leaves + b;
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  int32_t code = TREE_EMPTY;
+  if (b < n_bv && ranges[b].y > 0) {
+    lo = boxes[2 * b];
+    hi = boxes[2 * b + 1];
+    code = 0;
+  }
+  lo.w = __int_as_float(code);
+  hi.w = 0.f;
+  nodes[2 * n] = lo;
+  nodes[2 * n + 1] = hi;
+}
+
+// one tree level: node n <- union of its non-empty children, with the split code
+__global__ void k_tree_level(int32_t level, float4* __restrict__ nodes) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1 << level)) return;
+  const int32_t n = (1 << level) + i, a = 2 * n, b = 2 * n + 1;
+  const float4 alo = nodes[2 * a], ahi = nodes[2 * a + 1], blo = nodes[2 * b], bhi = nodes[2 * b + 1];
+  const bool ea = (__float_as_int(alo.w) & TREE_EMPTY) != 0, eb = (__float_as_int(blo.w) & TREE_EMPTY) != 0;
+  float4 lo, hi;
+  int32_t code;
+  if (ea && eb) {
+    lo = alo; hi = ahi; code = TREE_EMPTY;
+  } else if (ea || eb) {
+    lo = ea ? blo : alo; hi = ea ? bhi : ahi; code = 0;
+  } else {
+    lo = make_float4(fminf(alo.x, blo.x), fminf(alo.y, blo.y), fminf(alo.z, blo.z), 0.f);
+    hi = make_float4(fmaxf(ahi.x, bhi.x), fmaxf(ahi.y, bhi.y), fmaxf(ahi.z, bhi.z), 0.f);
+    // centres (doubled) of the children along each axis
+    const float dx = (blo.x + bhi.x) - (alo.x + ahi.x), dy = (blo.y + bhi.y) - (alo.y + ahi.y),
+                dz = (blo.z + bhi.z) - (alo.z + ahi.z);
+    int32_t axis = 0;
+    float d = dx;
+    if (fabsf(dy) > fabsf(d)) { axis = 1; d = dy; }
+    if (fabsf(dz) > fabsf(d)) { axis = 2; d = dz; }
+    code = axis | (d < 0.f ? TREE_SWAP : 0);
+  }
+  lo.w = __int_as_float(code);
+  hi.w = 0.f;
+  nodes[2 * n] = lo;
+  nodes[2 * n + 1] = hi;
 }
 
 // ---- launchers -------------------------------------------------------------------------------
@@ -247,6 +298,14 @@ void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
   else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, s);
 }
 
+void launch_tree_build(const float4* boxes, const int2* ranges, int32_t n_bv, int32_t levels, float4* nodes,
+                       hipStream_t s) {
+  const int32_t n_leaves = 1 << levels;
+  hipLaunchKernelGGL(k_tree_leaves, dim3((n_leaves + 255) / 256), dim3(256), 0, s, boxes, ranges, n_bv, levels, nodes);
+  for (int32_t l = levels - 1; l >= 0; --l)
+    hipLaunchKernelGGL(k_tree_level, dim3(((1 << l) + 255) / 256), dim3(256), 0, s, l, nodes);
+}
+
 void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t gsize, float4* groups, hipStream_t s) {
   const int32_t n_groups = (n_bv + gsize - 1) / gsize;
   hipLaunchKernelGGL(k_group_boxes, dim3((n_groups + 63) / 64), dim3(64), 0, s, boxes, n_bv, gsize, groups);
@@ -272,11 +331,13 @@ void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
   dim3 grid((unsigned)((max_hit_items((R + 63) / 64) + 3) / 4)), block(256);
-  const bool g = scene.groups != nullptr && scene.gsize > 0 && scene.n_groups > 0;
-  if (st && g) hipLaunchKernelGGL((k_hit_bv<true, true>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
-  else if (st) hipLaunchKernelGGL((k_hit_bv<true, false>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
-  else if (g) hipLaunchKernelGGL((k_hit_bv<false, true>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
-  else hipLaunchKernelGGL((k_hit_bv<false, false>), grid, block, 0, s, rays, R, d_count, scene, keys, st);
+#define SPL_HIT_BV(ST, M) hipLaunchKernelGGL((k_hit_bv<ST, M>), grid, block, 0, s, rays, R, d_count, scene, keys, st)
+  switch (bv_mode(scene)) {
+    case BV_TREE: if (st) SPL_HIT_BV(true, BV_TREE); else SPL_HIT_BV(false, BV_TREE); break;
+    case BV_GROUPS: if (st) SPL_HIT_BV(true, BV_GROUPS); else SPL_HIT_BV(false, BV_GROUPS); break;
+    default: if (st) SPL_HIT_BV(true, BV_FLAT); else SPL_HIT_BV(false, BV_FLAT); break;
+  }
+#undef SPL_HIT_BV
 }
 
 }  // namespace spl

@@ -58,10 +58,17 @@ __device__ __forceinline__ float key_t(uint64_t k) {
   return __uint_as_float((m >> 31) == 0 ? ~m : (m ^ 0x80000000u));
 }
 
-// One wave, one ray per lane: the lexicographic (t, index) minimum over the scene's BV leaves,
-// visited in order, optionally under the coarse level of groups of `gsize` consecutive leaves.
-// A box no lane enters (one ballot) is skipped with its members; otherwise all lanes test its
-// member records (a triangle held by two leaves ties with itself). Must be called by whole waves.
+// One wave, one ray per lane: the lexicographic (t, index) minimum over the scene's BV leaves.
+// The wave walks the top structure together: a box no lane enters (one ballot) is skipped with
+// everything under it; a leaf some lane enters has all its member records tested by every lane
+// (a triangle held by two leaves ties with itself). MODE (BVMode) picks the top structure:
+//   BV_FLAT   -- every leaf box in order (the reference's one-level DPBVHitter);
+//   BV_GROUPS -- group boxes of gsize consecutive leaves, then their leaves;
+//   BV_TREE   -- depth-first over the binary tree, nearer child first along the node's split axis
+//                for the wave's first live ray, without a stack (heap indices: up = n / 2).
+// Work split `split` of `n_split` takes groups/leaves split, split + n_split, ... (flat, groups)
+// or the subtree under node n_split + split (tree; n_split a power of two). Must be called by
+// whole waves.
 // Scene data is read-only for the whole launch: reading it through the constant address space
 // lets every wave-uniform access become a scalar load, whatever the kernel writes elsewhere.
 // (The host pass only parses these functions; it has no such address space.)
@@ -77,15 +84,13 @@ __device__ __forceinline__ cptr<T> as_const(const T* p) {
   return (cptr<T>)(p);
 }
 
-template <bool STATS, bool GROUPS>
+template <bool STATS, int MODE>
 __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, const BVScene& S, HitStats& st,
                                                float* best_io = nullptr, int32_t split = 0, int32_t n_split = 1,
                                                const uint64_t* shared_key = nullptr) {
   const cptr<float4> boxes = as_const(S.boxes);
   const cptr<int2> ranges = as_const(S.ranges);
   const cptr<TriRec> recs = as_const(S.recs);
-  const cptr<float4> groups = as_const(S.groups);
-  const int32_t n_bv = S.n_bv, gsize = S.gsize, n_groups = S.n_groups;
   unsigned long long& n_box = st.n_box;
   unsigned long long& n_enter = st.n_enter;
   unsigned long long& n_alg = st.n_alg;
@@ -108,16 +113,12 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       }
     }
   };
-
-  auto scan_leaf = [&](int32_t b) {
-    const float4 blo = boxes[2 * b], bhi = boxes[2 * b + 1];
-    const bool enter = valid && slab_enter(blo, bhi, L, ix, iy, iz, best);
-    if constexpr (STATS) n_box += valid ? 1ull : 0ull;
-    if (__builtin_amdgcn_ballot_w64(enter) == 0) return;
+  // every lane tests the members of leaf b (some lane entered it)
+  auto scan_members = [&](int32_t b, bool entered) {
     const int2 rg = ranges[b];
     if constexpr (STATS) {
-      n_enter += enter ? 1ull : 0ull;
-      n_alg += enter ? (unsigned long long)rg.y : 0ull;
+      n_enter += entered ? 1ull : 0ull;
+      n_alg += entered ? (unsigned long long)rg.y : 0ull;
       n_exec += (unsigned long long)rg.y;
     }
 #pragma unroll 4
@@ -132,8 +133,17 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       }
     }
   };
+  auto scan_leaf = [&](int32_t b) {
+    const float4 blo = boxes[2 * b], bhi = boxes[2 * b + 1];
+    const bool enter = valid && slab_enter(blo, bhi, L, ix, iy, iz, best);
+    if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+    if (__builtin_amdgcn_ballot_w64(enter) == 0) return;
+    scan_members(b, enter);
+  };
 
-  if constexpr (GROUPS) {
+  if constexpr (MODE == BV_GROUPS) {
+    const cptr<float4> groups = as_const(S.groups);
+    const int32_t n_bv = S.n_bv, gsize = S.gsize, n_groups = S.n_groups;
     for (int32_t g = split; g < n_groups; g += n_split) {
       refresh();
       const float4 glo = groups[2 * g], ghi = groups[2 * g + 1];
@@ -143,7 +153,58 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       const int32_t b1 = (g + 1) * gsize < n_bv ? (g + 1) * gsize : n_bv;
       for (int32_t b = g * gsize; b < b1; ++b) scan_leaf(b);
     }
+  } else if constexpr (MODE == BV_TREE) {
+    const cptr<float4> nodes = as_const(S.nodes);
+    const int32_t leaf0 = 1 << S.levels;
+    // splits beyond the leaf count have no subtree of their own
+    const int32_t n_sub = n_split < leaf0 ? n_split : leaf0;
+    const uint64_t live = __builtin_amdgcn_ballot_w64(valid);
+    if (live != 0 && split < n_sub) {
+      // direction signs of the wave's first live ray order the children
+      const int32_t fl = (int32_t)__builtin_ctzll(live);
+      const float rdx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dx), fl));
+      const float rdy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dy), fl));
+      const float rdz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dz), fl));
+      const int32_t neg = (rdx < 0.f ? 1 : 0) | (rdy < 0.f ? 2 : 0) | (rdz < 0.f ? 4 : 0);
+      auto first_child = [&](int32_t p, int32_t code) {
+        return 2 * p + (((neg >> (code & TREE_AXIS)) & 1) ^ ((code & TREE_SWAP) ? 1 : 0));
+      };
+      // the subtree of this split (n_split is a power of two)
+      const int32_t root = n_sub + split;
+      int32_t n = root;
+      for (;;) {
+        const float4 lo = nodes[2 * n], hi = nodes[2 * n + 1];
+        const int32_t code = __float_as_int(lo.w);
+        bool descend = false;
+        if (!(code & TREE_EMPTY)) {
+          const bool enter = valid && slab_enter(lo, hi, L, ix, iy, iz, best);
+          if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+          if (__builtin_amdgcn_ballot_w64(enter) != 0) {
+            if (n < leaf0) {
+              descend = true;
+            } else {
+              refresh();
+              scan_members(n - leaf0, enter);
+            }
+          }
+        }
+        if (descend) {
+          n = first_child(n, code);
+          continue;
+        }
+        // next node: climb while n is its parent's second child
+        bool done = false;
+        for (;;) {
+          if (n == root) { done = true; break; }
+          const int32_t p = n >> 1;
+          if (n == first_child(p, __float_as_int(nodes[2 * p].w))) { n ^= 1; break; }
+          n = p;
+        }
+        if (done) break;
+      }
+    }
   } else {
+    const int32_t n_bv = S.n_bv;
     for (int32_t b = split; b < n_bv; b += n_split) {
       if (((b - split) & 15) == 0) refresh();
       scan_leaf(b);
@@ -153,8 +214,8 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
   return bi;
 }
 
-// Split factor for `groups64` live 64-ray groups over a scene with `units` top-level boxes:
-// enough (group, split) items to fill the chip, never more splits than boxes.
+// Split factor for `groups64` live 64-ray groups over a scene with `units` top-level boxes (or
+// tree leaves): enough (group, split) items to fill the chip, never more splits than units.
 constexpr int64_t SPLIT_TARGET_ITEMS = 16384;
 constexpr int32_t MAX_SPLIT = 32;
 __device__ __forceinline__ int32_t split_factor(int64_t groups64, int32_t units) {
@@ -171,7 +232,7 @@ inline __host__ __device__ int64_t max_hit_items(int64_t groups64) {
 // One (64-ray group, split) work item: rays [base, base + 64) of `rays` with A live, scanning
 // split `split` of `n_split`; the result is merged into keys[r] with a 64-bit atomic minimum of
 // the packed (t, index) key. Called by whole waves.
-template <bool STATS, bool GROUPS>
+template <bool STATS, int MODE>
 __device__ __forceinline__ void bv_item(const splat_adray_t* __restrict__ rays, int64_t A, int64_t base, int32_t split,
                                         int32_t n_split, const BVScene& scene, uint64_t* __restrict__ keys,
                                         HitStats& st) {
@@ -179,7 +240,7 @@ __device__ __forceinline__ void bv_item(const splat_adray_t* __restrict__ rays, 
   const bool valid = r < A;
   const RayLanes L = load_ray_lanes(rays, r, valid);
   float best;
-  const int32_t bi = bv_traverse<STATS, GROUPS>(L, valid, scene, st, &best, split, n_split,
+  const int32_t bi = bv_traverse<STATS, MODE>(L, valid, scene, st, &best, split, n_split,
                                                 n_split > 1 && valid ? keys + r : nullptr);
   if (valid) {
     const uint64_t k = pack_key(best, bi);

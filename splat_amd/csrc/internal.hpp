@@ -18,15 +18,48 @@ int choose_bf_waves(int64_t R, int32_t N);
 void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
                    int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s);
 void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, float4* groups, hipStream_t s);
+// Binary tree over the BVs in their order, heap-indexed: node n has children 2n and 2n + 1, the
+// 2^levels leaves are nodes 2^levels + b (BV b, or empty padding). nodes[2n] / nodes[2n + 1] are
+// the node's box; nodes[2n].w holds a code: bits 0-1 = the axis that separates the children's
+// centres most, bit 2 = the right child lies lower on it, bit 3 = empty subtree.
+constexpr int32_t TREE_AXIS = 3, TREE_SWAP = 4, TREE_EMPTY = 8;
+constexpr int32_t MAX_TREE_LEVELS = 24;
+void launch_tree_build(const float4* boxes, const int2* ranges, int32_t n_bv, int32_t levels, float4* nodes,
+                       hipStream_t s);
 
-// Prepared scene of the BV hitter as the kernels see it.
+// Prepared scene of the BV hitter as the kernels see it: the leaves (boxes, member ranges,
+// records) under one of three top structures -- none (every leaf box tested), a coarse level of
+// groups of gsize consecutive leaves, or a binary tree over the leaves (nodes != nullptr).
 struct BVScene {
   const float4* boxes;
   const int2* ranges;
   const TriRec* recs;
   const float4* groups;
   int32_t n_bv, gsize, n_groups;
+  const float4* nodes;
+  int32_t levels;
 };
+enum BVMode { BV_FLAT = 0, BV_GROUPS = 1, BV_TREE = 2 };
+// the kernels' view of a prepared BV hit set
+inline BVScene bv_scene(const splat_hitset_t* hs) {
+  BVScene b;
+  b.boxes = static_cast<const float4*>(hs->d_boxes);
+  b.ranges = static_cast<const int2*>(hs->d_ranges);
+  b.recs = static_cast<const TriRec*>(hs->d_recs);
+  b.n_bv = hs->n_bv;
+  const bool tree = hs->d_groups && hs->tree_levels > 0;
+  const bool g = !tree && hs->d_groups && hs->group_size > 0 && hs->n_groups > 0;
+  b.groups = g ? static_cast<const float4*>(hs->d_groups) : nullptr;
+  b.gsize = g ? hs->group_size : 0;
+  b.n_groups = g ? hs->n_groups : 0;
+  b.nodes = tree ? static_cast<const float4*>(hs->d_groups) : nullptr;
+  b.levels = tree ? hs->tree_levels - 1 : 0;
+  return b;
+}
+
+inline __host__ __device__ int bv_mode(const BVScene& s) {
+  return s.nodes ? BV_TREE : (s.groups && s.gsize > 0 && s.n_groups > 0 ? BV_GROUPS : BV_FLAT);
+}
 // BV hitter into packed keys (initialised by launch_key_init on the same stream)
 void launch_key_init(uint64_t* keys, int64_t R, const int64_t* d_count, hipStream_t s);
 void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const BVScene& scene, uint64_t* keys,

@@ -27,8 +27,8 @@ class ResidentScene:
 
     @staticmethod
     def build(sc: scene.Scene, depth: int | None = None, device: str = "cuda") -> "ResidentScene":
-        if depth is None:  # ~40 triangles per leaf
-            depth = max(2, min(12, int(round(math.log2(max(sc.n_tris, 2) / 40)))))
+        if depth is None:
+            depth = bv.tree_depth(sc.n_tris)
         tris = api.to_device(sc.ftris, device)
         bvs, mem = bv.object_partition(sc.ftris, depth)
         return ResidentScene(sc, tris, api.HIPBVHitter(tris, bvs, mem, group_size=0), sc.name)

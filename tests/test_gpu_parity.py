@@ -124,8 +124,8 @@ def test_next_hit_d_count_limits_rays(c1):
 
 
 @pytest.mark.parametrize("build,depth,group_size", [
-    ("reference", 0, 0), ("reference", 2, -1), ("reference", 4, 0), ("reference", 6, 3),
-    ("object", 6, -1), ("object", 8, 0), ("object", 8, 5),
+    ("reference", 0, 0), ("reference", 2, -1), ("reference", 4, 0), ("reference", 6, 3), ("reference", 6, 0),
+    ("object", 6, -1), ("object", 8, 0), ("object", 8, 5), ("object", 1, 0), ("object", 3, 0), ("object", 11, 0),
 ])
 def test_next_hit_bv_equals_bf(build, depth, group_size):
     sc = H.small_c2(3)
