@@ -75,31 +75,51 @@ def build_workload(device="cuda"):
     return sc, cam, (bvs, mem), hitter, params, lm, frame
 
 
-def cpu_baseline(sc, cam, bvmem, params, lm, budget_s: float = 12.0):
+def live_counts(cam, hitter, tris, params) -> list[int]:
+    """Live (ACTIVE) rays before each bounce of the frame, A_0 = R, ..., A_depth, from the same
+    hitter / evolve / partition operators stepped one bounce at a time (identical rays and
+    random numbers to the fused loop)."""
+    from splat_amd import api
+
+    rays = api.rays_from_pinhole(cam)
+    tr = api.HIPTracer(rays)
+    out = [int(rays.shape[0])]
+    for k in range(1, params.depth + 1):
+        api.next_hit_(tr, hitter, rays)
+        api.evolve_(rays, tr.hit_idx, tris, first_diffuse=params.first_diffuse, seed=params.seed, bounce=k,
+                    optics=params.optics)
+        out.append(api.partition_(rays, tr.ray_swap, tr))
+    return out
+
+
+def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
     """The oracle's FP32 C++ restatement of the same frame (BV-culled hitter, OpenMP), timed on
     this host on an image-wide strided sample of the frame's rays."""
     from oracle import oracle as O
     from splat_amd import bv
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    counts, mat, stride_m = bv.pack_bv_tris(bvmem[1], sc.n_tris)
+    # the CPU port walks the leaves flat (the reference's one-level BV hitter): ~sqrt(N) leaves
+    # balance its box and triangle tests
+    depth = max(1, int(round(0.5 * math.log2(max(sc.n_tris, 2)))))
+    counts, mat, stride_m = bv.pack_bv_tris(bv.object_partition(sc.ftris, depth)[1], sc.n_tris)
     R_full = cam.height * cam.width * cam.samples_per_pixel
     t = time.perf_counter()
-    _, n = O.trace_sample(cam, 512, 0, sc.ftris, params, lm, (counts, mat, stride_m), threads)
+    _, n = O.trace_sample(cam, 64, 0, sc.ftris, params, lm, (counts, mat, stride_m), threads)
     per_ray = (time.perf_counter() - t) / max(n, 1)
     # a strided sample of about budget_s seconds; a frame cheaper than that is traced repeatedly
     stride = max(1, int(math.ceil(R_full * per_ray / budget_s)))
-    reps = 1 if stride > 1 else max(1, min(100, int(budget_s / max(R_full * per_ray, 1e-9))))
-    n = 0
+    n, reps = 0, 0
     t = time.perf_counter()
-    for _ in range(reps):
+    while reps == 0 or (stride == 1 and time.perf_counter() - t < budget_s and reps < 100):
         n += O.trace_sample(cam, stride, 0, sc.ftris, params, lm, (counts, mat, stride_m), threads)[1]
+        reps += 1
     dt = time.perf_counter() - t
     what = f"every {stride}th ray of the {cam.width}x{cam.height} frame" if stride > 1 else \
         f"the whole {cam.width}x{cam.height} frame x {reps}"
     return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{what} ({n} rays, {dt:.1f} s): oracle FP32 C++ frame (same BV culling, "
-                      f"{params.depth} bounces, splat), OpenMP {threads} threads"}
+            "sample": f"{what} ({n} rays, {dt:.1f} s): oracle FP32 C++ frame (flat BV culling over "
+                      f"2^{depth} object-median leaves, {params.depth} bounces, splat), OpenMP {threads} threads"}
 
 
 def batched_vs_plain(reps: int = 3):
@@ -140,19 +160,58 @@ def batched_vs_plain(reps: int = 3):
             "plain_s": round(tp, 4), "batched_s": round(tb, 4), "steps_per_batch": st.steps}
 
 
-def served(n_requests: int = 100, rate: float = 12.0):
+def served(n_requests: int = 120, rate: float = 40.0):
     """Poisson arrivals over 8 resident scenes (320 to ~178k triangles), resolution uniform over
-    256^2-2048^2, 1-16 stratified wavelengths per pixel, 4 bounces, admitted against an HBM budget."""
+    256^2-2048^2, 1-16 stratified wavelengths per pixel, 4 bounces, admitted against an HBM budget
+    and continuously batched into shared steps."""
+    import numpy as np
+
     from splat_amd import api, serve
 
     scenes = serve.serving_scenes()
     reqs = serve.serving_requests(n_requests, rate, len(scenes), [s.sc.first_diffuse for s in scenes])
     eng = api.Engine(96 << 30, max_slots=64)
+    serve.run_requests(eng, scenes, reqs[:4], realtime=False)  # warm-up: graphs, first-touch
+    reqs = serve.serving_requests(n_requests, rate, len(scenes), [s.sc.first_diffuse for s in scenes])
     st = serve.run_requests(eng, scenes, reqs, realtime=True)
     eng.close()
     out = st.summary()
+    small = np.array([(rq.done - rq.arrival) * 1e3 for rq in reqs if rq.rays <= (1 << 20)])
     out.update({"arrival_rate_per_s": rate, "hbm_budget_gib": 96,
+                "offered_mrays_per_s": round(sum(rq.rays for rq in reqs) / reqs[-1].arrival / 1e6, 1),
+                "p99_ms_requests_upto_1M_rays": round(float(np.percentile(small, 99)), 3) if small.size else None,
                 "triangles": [s.sc.n_tris for s in scenes]})
+    return out
+
+
+def served_1mp(n_requests: int = 300, rate: float = 400.0):
+    """The latency case: Poisson arrivals of 1 MP requests (1024^2 rays, one seeded wavelength per
+    ray, 4 bounces, 1024^2 RGB light map) over the 10,240-triangle two-icosphere scene, each with
+    its own pose and light, continuously batched; p50/p99 = completion - arrival."""
+    import math
+
+    from splat_amd import abi, api, scene, serve
+
+    sc = scene.scene_c2(seed=2)
+    rs = serve.ResidentScene.build(sc)
+    arr = serve.poisson_arrivals(n_requests, rate, seed=7)
+
+    def make():
+        reqs = []
+        for k in range(n_requests):
+            cam = serve.lights_cam(100 + k, 1024, 1, abi.LAMBDA_UNIFORM)
+            p = abi.run_params(4, sc.first_diffuse, math.nan, seed=1000 + k)
+            reqs.append(serve.Request(k, 0, cam, p, abi.lightmap_params(1024, 1024), arrival=float(arr[k])))
+        return reqs
+
+    eng = api.Engine(16 << 30, max_slots=64)
+    serve.run_requests(eng, [rs], make()[:8], realtime=False)  # warm-up
+    reqs = make()
+    st = serve.run_requests(eng, [rs], reqs, realtime=True)
+    eng.close()
+    out = st.summary()
+    out.update({"arrival_rate_per_s": rate, "rays_per_request": 1 << 20, "triangles": sc.n_tris,
+                "offered_mrays_per_s": round(n_requests * (1 << 20) / float(arr[-1]) / 1e6, 1)})
     return out
 
 
@@ -165,6 +224,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only a 256 MiB device-to-device copy, repeated: calibrates HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -178,6 +239,17 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        n = 256 << 20
+        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(max(args.steps, 1)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": max(args.steps, 1)}))
+        return 0
+    from splat_amd import api
+
     sc, cam, bvmem, hitter, params, lm, frame = build_workload()
     R = frame.R
     for _ in range(args.warmup):
@@ -200,22 +272,37 @@ def main() -> int:
         print(json.dumps({"frames": K, "ms_per_step": wall / K * 1e3}))
         return 0
 
-    # ---- instrumented frames: hitter launches timed with events on the frame's stream ----
+    # ---- instrumented frames: hitter, partition-scatter and splat launches timed with events
+    # recorded on the frame's own stream around each launch ----
     hip = Hip()
-    n_launch = params.depth + 1
+    D = params.depth
+    n_launch = D + 1
     M = 20
-    hev = hip.events(2 * n_launch * M)
-    per = 2 * n_launch
+    per = 4 * D + 4
+    hev = hip.events(per * M)
     for m in range(M):
         params.hit_events = C.addressof(hev) + m * per * C.sizeof(C.c_void_p)
         params.n_hit_events = per
         frame.trace()
     torch.cuda.synchronize()
-    launch_ms = [hip.elapsed_ms(hev[m * per + 2 * j], hev[m * per + 2 * j + 1]) for m in range(M) for j in range(n_launch)]
+    ms = lambda m, i: hip.elapsed_ms(hev[m * per + i], hev[m * per + i + 1])  # noqa: E731
+    launch_ms = [ms(m, 2 * j) for m in range(M) for j in range(n_launch)]
+    part_ms = [[ms(m, 2 * n_launch + 2 * j) for j in range(D)] for m in range(M)]
+    splat_ms = [ms(m, 4 * D + 2) for m in range(M)]
     params.hit_events, params.n_hit_events = None, 0
     hip.destroy(hev)
     avg_launch_s = float(np.mean(launch_ms)) / 1e3
     hitter_frame_ms = float(np.sum(launch_ms)) / M
+
+    # ---- algorithmic bytes of the partition scatter and the splat ----
+    live = live_counts(cam, hitter, frame.tris, params)  # A_0 = R, A_1, ..., A_D
+    fin = api.from_device(frame.rays, abi.ADRAY)
+    n_diffuse = int((fin["status"] == abi.RAY_DIFFUSE).sum())
+    part_bytes = sum(272 * live[k] + 12 * (live[k] - live[k + 1]) for k in range(D))
+    part_s = float(np.mean([sum(x) for x in part_ms])) / 1e3
+    tex_bytes = lm.width * lm.height * lm.channels * 4
+    splat_bytes = R * (136 + 4) + n_diffuse * 136 + 2 * tex_bytes + sc.n_tris * 120
+    splat_s = float(np.mean(splat_ms)) / 1e3
 
     # ---- algorithmic work of the hitter (one counting frame, untimed) ----
     stats = torch.zeros(4, dtype=torch.int64, device="cuda")
@@ -232,7 +319,9 @@ def main() -> int:
     batched = None if args.no_batched else batched_vs_plain()
     torch.cuda.empty_cache()
     serve_out = None if args.no_served else served()
-    cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, bvmem, params, lm)
+    torch.cuda.empty_cache()
+    serve_1mp = None if args.no_served else served_1mp()
+    cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, params, lm)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -268,6 +357,15 @@ def main() -> int:
             "flop_per_launch": flop_per_launch,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
+        "rooflines": {
+            "partition_scatter": {"bound": "hbm", "achieved": round(part_bytes / part_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(part_bytes / part_s / 1e9 / HBM_PEAK_GBS, 4),
+                                  "bytes_per_frame": part_bytes, "ms_per_frame": round(part_s * 1e3, 4),
+                                  "live_before_bounce": live},
+            "splat": {"bound": "hbm", "achieved": round(splat_bytes / splat_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(splat_bytes / splat_s / 1e9 / HBM_PEAK_GBS, 4),
+                      "bytes_per_frame": splat_bytes, "ms_per_frame": round(splat_s * 1e3, 4), "diffuse_rays": n_diffuse},
+        },
         "hitter": {
             "ms_per_frame": round(hitter_frame_ms, 4),
             "box_tests_per_frame": box_tests,
@@ -277,6 +375,7 @@ def main() -> int:
         },
         "batched": batched,
         "served": serve_out,
+        "served_1mp": serve_1mp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

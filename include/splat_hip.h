@@ -189,9 +189,11 @@ typedef struct splat_run_params {
   int32_t _pad;
   uint64_t seed;         /* bounce random numbers: f(seed, bounce, ray.dest)           */
   splat_optics_t optics;
-  /* Optional instrumentation: 2*(depth+1) hipEvent_t handles (as void*), recorded on the
-   * stream immediately before and after each hitter launch of the loop (bounces 1..depth, then
-   * the realigning pass). NULL records nothing. */
+  /* Optional instrumentation: hipEvent_t handles (as void*) recorded on the stream immediately
+   * before and after launches: [0, 2*(depth+1)) each hitter launch (bounces 1..depth, then the
+   * realigning pass); with 4*depth+4 handles also [2*(depth+1), 4*depth+2) each bounce's
+   * partition scatter and [4*depth+2, 4*depth+4) the light map (splat_trace_frame). NULL or
+   * fewer handles record nothing (resp. only the hitter launches). */
   void** hit_events;
   int32_t n_hit_events;
   int32_t _pad2;

@@ -159,7 +159,11 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
                  int32_t* hit_final, const splat_run_params_t* p, const RunWs& w, hipStream_t s) {
   const int64_t n_tiles = (R + TILE - 1) / TILE;
   const bool ev = p->hit_events && p->n_hit_events >= 2 * (p->depth + 1);
+  const bool ev_all = p->hit_events && p->n_hit_events >= 4 * p->depth + 4;
   auto mark = [&](int i) { if (ev) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[i]), s); };
+  auto mark_part = [&](int i) {
+    if (ev_all) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[2 * (p->depth + 1) + i]), s);
+  };
   launch_set_i64(&w.counters[0], R, s);
   splat_adray_t* cur = w.s1;
   splat_adray_t* nxt = w.s2;
@@ -172,7 +176,9 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
     launch_evolve(cur, R, live, nullptr, w.keys, tris, hs->n_tris, nullptr, p->force_rand, p->seed, k, p->first_diffuse, p->optics,
                   w.tile_counts, s);
     launch_scan_tiles(w.tile_counts, n_tiles, live, R, w.tile_offsets, &w.counters[k], s);
+    mark_part(2 * (k - 1));
     launch_scatter(cur, R, live, w.tile_offsets, &w.counters[k], nxt, final_rays, nullptr, w.keys, hit_final, s);
+    mark_part(2 * (k - 1) + 1);
     splat_adray_t* t = cur;
     cur = nxt;
     nxt = t;
@@ -465,7 +471,10 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
   launch_fill_f32(tex, (int64_t)lm->width * lm->height * lm->channels, 0.0f, s);
   launch_raygen(*cam, w.s1, R, s);
   if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s)) return e;
+  const bool ev_all = p->hit_events && p->n_hit_events >= 4 * p->depth + 4;
+  if (ev_all) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[4 * p->depth + 2]), s);
   launch_light_map(rays, R, nullptr, hit_idx, tris, N, *lm, tex, s);
+  if (ev_all) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[4 * p->depth + 3]), s);
   return launch_status();
 }
 
