@@ -272,13 +272,13 @@ def main() -> int:
         print(json.dumps({"frames": K, "ms_per_step": wall / K * 1e3}))
         return 0
 
-    # ---- instrumented frames: hitter, partition-scatter and splat launches timed with events
-    # recorded on the frame's own stream around each launch ----
+    # ---- instrumented frames: hitter and fused bounce-pass launches timed with events recorded
+    # on the frame's own stream around each launch ----
     hip = Hip()
     D = params.depth
     n_launch = D + 1
     M = 20
-    per = 4 * D + 4
+    per = 4 * D + 2
     hev = hip.events(per * M)
     for m in range(M):
         params.hit_events = C.addressof(hev) + m * per * C.sizeof(C.c_void_p)
@@ -287,22 +287,23 @@ def main() -> int:
     torch.cuda.synchronize()
     ms = lambda m, i: hip.elapsed_ms(hev[m * per + i], hev[m * per + i + 1])  # noqa: E731
     launch_ms = [ms(m, 2 * j) for m in range(M) for j in range(n_launch)]
-    part_ms = [[ms(m, 2 * n_launch + 2 * j) for j in range(D)] for m in range(M)]
-    splat_ms = [ms(m, 4 * D + 2) for m in range(M)]
+    pass_ms = [[ms(m, 2 * n_launch + 2 * j) for j in range(D)] for m in range(M)]
     params.hit_events, params.n_hit_events = None, 0
     hip.destroy(hev)
     avg_launch_s = float(np.mean(launch_ms)) / 1e3
     hitter_frame_ms = float(np.sum(launch_ms)) / M
 
-    # ---- algorithmic bytes of the partition scatter and the splat ----
+    # ---- algorithmic bytes of the fused bounce passes (evolve + splat + stable partition): per
+    # live ray its key (8 B), the ray read (136 B; not at the first bounce, which generates its
+    # rays) and written (136 B) once, the hit index of each retiring ray (4 B); per frame the light
+    # map read and written once and every triangle gathered once ----
     live = live_counts(cam, hitter, frame.tris, params)  # A_0 = R, A_1, ..., A_D
     fin = api.from_device(frame.rays, abi.ADRAY)
     n_diffuse = int((fin["status"] == abi.RAY_DIFFUSE).sum())
-    part_bytes = sum(272 * live[k] + 12 * (live[k] - live[k + 1]) for k in range(D))
-    part_s = float(np.mean([sum(x) for x in part_ms])) / 1e3
     tex_bytes = lm.width * lm.height * lm.channels * 4
-    splat_bytes = R * (136 + 4) + n_diffuse * 136 + 2 * tex_bytes + sc.n_tris * 120
-    splat_s = float(np.mean(splat_ms)) / 1e3
+    pass_bytes = sum((144 if k == 0 else 280) * live[k] + 4 * (live[k] - live[k + 1]) for k in range(D)) \
+        + 2 * tex_bytes + sc.n_tris * 120
+    pass_s = float(np.mean([sum(x) for x in pass_ms])) / 1e3
 
     # ---- algorithmic work of the hitter (one counting frame, untimed) ----
     stats = torch.zeros(4, dtype=torch.int64, device="cuda")
@@ -358,13 +359,11 @@ def main() -> int:
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
         "rooflines": {
-            "partition_scatter": {"bound": "hbm", "achieved": round(part_bytes / part_s / 1e9, 1), "peak": HBM_PEAK_GBS,
-                                  "unit": "GB/s", "frac": round(part_bytes / part_s / 1e9 / HBM_PEAK_GBS, 4),
-                                  "bytes_per_frame": part_bytes, "ms_per_frame": round(part_s * 1e3, 4),
-                                  "live_before_bounce": live},
-            "splat": {"bound": "hbm", "achieved": round(splat_bytes / splat_s / 1e9, 1), "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": round(splat_bytes / splat_s / 1e9 / HBM_PEAK_GBS, 4),
-                      "bytes_per_frame": splat_bytes, "ms_per_frame": round(splat_s * 1e3, 4), "diffuse_rays": n_diffuse},
+            "bounce_pass": {"kernel": "k_bounce (evolve + splat + stable partition, fused)", "bound": "hbm",
+                            "achieved": round(pass_bytes / pass_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(pass_bytes / pass_s / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_frame": pass_bytes,
+                            "ms_per_frame": round(pass_s * 1e3, 4), "live_before_bounce": live,
+                            "diffuse_rays": n_diffuse},
         },
         "hitter": {
             "ms_per_frame": round(hitter_frame_ms, 4),

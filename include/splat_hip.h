@@ -191,9 +191,9 @@ typedef struct splat_run_params {
   splat_optics_t optics;
   /* Optional instrumentation: hipEvent_t handles (as void*) recorded on the stream immediately
    * before and after launches: [0, 2*(depth+1)) each hitter launch (bounces 1..depth, then the
-   * realigning pass); with 4*depth+4 handles also [2*(depth+1), 4*depth+2) each bounce's
-   * partition scatter and [4*depth+2, 4*depth+4) the light map (splat_trace_frame). NULL or
-   * fewer handles record nothing (resp. only the hitter launches). */
+   * realigning pass); with 4*depth+2 handles also [2*(depth+1), 4*depth+2) each bounce's fused
+   * pass (evolve, splat of the rays that become DIFFUSE, stable partition). NULL or fewer
+   * handles record nothing (resp. only the hitter launches). */
   void** hit_events;
   int32_t n_hit_events;
   int32_t _pad2;

@@ -1,0 +1,155 @@
+// bounce.hpp -- one bounce of one tile of live rays in a single pass over the ray data, shared by
+// the single-frame loop (trace.hip) and the multi-frame engine (engine.hip):
+//
+//   count pass (bounce_count):  per tile, how many rays stay ACTIVE -- from the hit keys alone
+//                               (status_after), so it reads 8 bytes per ray, not the ray;
+//   scan:                       tile offsets and the live count A_k (partition.hip);
+//   bounce pass (bounce_tile):  coalesced read of the tile into LDS -> evolve every ray in place
+//                               (dual-number refract/reflect) -> rays that became DIFFUSE get their
+//                               final evolution and splat into the light map at once -> stable
+//                               partition by ballot ranks -> coalesced writes of the two runs:
+//                               ACTIVE rays to the next live buffer, the others straight to their
+//                               final slots with their hit index.
+//
+// This replaces evolve (read + write every ray), scatter (read + write every ray) and the light
+// map's read of every final ray with one read and one write; at the first bounce the tile is
+// generated from the pinhole in LDS, so the rays are never written before their first bounce.
+// Positions come from status_after in both passes (same function of the same inputs), so they
+// always agree with the counts.
+#pragma once
+
+#include "hitcore.hpp"
+#include "path.hpp"
+#include "scatter.hpp"
+
+namespace spl {
+
+// What a bounce pass needs besides the rays and the light map.
+struct BounceParams {
+  const splat_ftri_t* tris;
+  int32_t n_tris, first_diffuse, bounce;
+  float force_rand;
+  uint64_t seed;
+  splat_optics_t optics;
+  float* tex;  // non-null: splat the rays that become DIFFUSE into this light map
+};
+
+__device__ __forceinline__ int32_t clamp_index(int32_t i, int32_t N) { return (i < 1 || i > N) ? 1 : i; }
+
+// ACTIVE rays after this bounce among the tile's first nvalid rays (a whole TILE-thread workgroup
+// calls it; thread 0 gets the count). all_active: every ray of the tile is known to be ACTIVE
+// (the loop's live rays), so no ray is read unless the sentinel test needs it. CAM: the rays are
+// the pinhole's rays base, base + 1, ... (not yet in memory).
+template <bool CAM>
+__device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in, const uint64_t* __restrict__ keys,
+                                            int nvalid, const BounceParams& P, bool all_active,
+                                            const splat_pinhole_t& cam, int64_t base) {
+  bool act = false;
+  const int t = threadIdx.x;
+  if (t < nvalid && (CAM || all_active || in[t].status == SPLAT_RAY_ACTIVE)) {
+    const int32_t i = clamp_index(key_index(keys[t]), P.n_tris);
+    bool tri1 = false;
+    if (i == 1 && P.first_diffuse > 1 && sentinel_live(P.tris)) {
+      if constexpr (CAM) {
+        const splat_adray_t g = raygen_ray(cam, base + t);
+        tri1 = sentinel_hit(P.tris, ld3(g.pos), ld3(g.dir), g.ignore_tri);
+      } else {
+        tri1 = sentinel_hit(P.tris, ld3(in[t].pos), ld3(in[t].dir), in[t].ignore_tri);
+      }
+    }
+    act = status_after(i, P.first_diffuse, tri1) == SPLAT_RAY_ACTIVE;
+  }
+  return __syncthreads_count(act);
+}
+
+// The bounce pass's LDS: one instance per workgroup, whichever bounce_tile variant runs.
+struct BounceLDS {
+  uint2 stage[TILE * WORDS];  // 34,816 bytes: the tile's records
+  int16_t order[TILE];
+  int32_t wave_a[TILE / 64], wave_r[TILE / 64];
+};
+__device__ __forceinline__ BounceLDS& bounce_lds() {
+  __shared__ BounceLDS lds;
+  return lds;
+}
+
+// The bounce pass of one tile (see the top of this file). off_a / off_r: where the tile's ACTIVE
+// and other rays start in out_active / out_rest; hit_rest gets the hit index of the others;
+// keys_next (nullable) gets the miss key (Inf, 1) at each ACTIVE ray's new position, ready for
+// the next bounce's hitter. CAM: the tile is generated from the pinhole (rays base, base + 1,
+// ...) instead of read from `in`.
+template <bool CAM>
+__device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in, const uint64_t* __restrict__ keys,
+                                            uint64_t* __restrict__ keys_next, int nvalid, const BounceParams& P,
+                                            const splat_lightmap_params_t& lm, const splat_pinhole_t& cam,
+                                            int64_t base, int64_t off_a, int64_t off_r,
+                                            splat_adray_t* __restrict__ out_active, splat_adray_t* __restrict__ out_rest,
+                                            int32_t* __restrict__ hit_rest) {
+  BounceLDS& lds = bounce_lds();
+  uint2* stage = lds.stage;
+  int16_t* order = lds.order;
+  int32_t* wave_a = lds.wave_a;
+  int32_t* wave_r = lds.wave_r;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool valid = tid < nvalid;
+  splat_adray_t* rec = reinterpret_cast<splat_adray_t*>(stage) + tid;
+  if constexpr (CAM) {
+    if (valid) *rec = raygen_ray(cam, base + tid);
+  } else {
+    const uint2* src = reinterpret_cast<const uint2*>(in);
+    for (int e = threadIdx.x; e < nvalid * WORDS; e += TILE) stage[e] = src[e];
+  }
+  __syncthreads();
+
+  int32_t i = 1;
+  uint8_t placed = 0;  // the status that decides where the ray goes
+  if (valid) {
+    RayF ray = load_rayf(rec);
+    if (ray.status == SPLAT_RAY_ACTIVE) {
+      i = clamp_index(key_index(keys[tid]), P.n_tris);
+      placed = predict_status(i, P.first_diffuse, P.tris, ray.pos, ray.dir, ray.ignore_tri);
+      const FTriF F = load_ftri(P.tris + (i - 1));
+      const float u = bounce_random(P.force_rand, nullptr, 0, P.seed, P.bounce, ray.dest);
+      RayF o = evolve_one(ray, i, F, u, P.first_diffuse, P.optics);
+      if (P.tex && o.status == SPLAT_RAY_DIFFUSE) final_evolve_splat(o, i, F, lm, P.tex);
+      store_rayf(rec, o);
+    } else {
+      placed = ray.status;
+    }
+  }
+  const bool act = valid && placed == SPLAT_RAY_ACTIVE;
+  const bool rest = valid && !act;
+  const uint64_t ma = __builtin_amdgcn_ballot_w64(act), mr = __builtin_amdgcn_ballot_w64(rest);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int ra = __builtin_popcountll(ma & below), rr = __builtin_popcountll(mr & below);
+  if (lane == 0) {
+    wave_a[wave] = __builtin_popcountll(ma);
+    wave_r[wave] = __builtin_popcountll(mr);
+  }
+  __syncthreads();
+  int pa = 0, pr = 0, na = 0;
+  for (int w = 0; w < TILE / 64; ++w) {
+    if (w < wave) { pa += wave_a[w]; pr += wave_r[w]; }
+    na += wave_a[w];
+  }
+  if (act) {
+    order[pa + ra] = (int16_t)tid;
+    if (keys_next) keys_next[off_a + pa + ra] = KEY_MISS;
+  }
+  if (rest) {
+    order[na + pr + rr] = (int16_t)tid;
+    hit_rest[off_r + pr + rr] = i;
+  }
+  __syncthreads();
+  uint2* dst_a = reinterpret_cast<uint2*>(out_active + off_a);
+  uint2* dst_r = reinterpret_cast<uint2*>(out_rest + off_r);
+  for (int e = threadIdx.x; e < nvalid * WORDS; e += TILE) {
+    const int slot = e / WORDS, w = e - slot * WORDS;
+    const uint2 v = stage[order[slot] * WORDS + w];
+    if (slot < na) dst_a[e] = v;
+    else dst_r[e - na * WORDS] = v;
+  }
+  __syncthreads();  // the LDS stage is reused by the caller's next tile
+}
+
+}  // namespace spl

@@ -2,9 +2,10 @@
 // the device-resident bounce loop.
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
-#include "internal.hpp"
+#include "bounce.hpp"
 
 using namespace spl;
 
@@ -36,7 +37,8 @@ constexpr int MAX_DEPTH = 64;
 struct RunWs {
   splat_adray_t* s1;
   splat_adray_t* s2;
-  uint64_t* keys;  // packed (t, index) nearest hit of each live ray
+  uint64_t* keys;  // packed (t, index) nearest hit of each live ray, ping-pong with keys2
+  uint64_t* keys2;
   int32_t* tile_counts;
   int32_t* tile_offsets;
   int64_t* counters;  // counters[k] = live rays after bounce k (counters[0] = R)
@@ -50,6 +52,7 @@ RunWs carve_run(void* ws, int64_t R) {
   w.s1 = c.take<splat_adray_t>((size_t)R);
   w.s2 = c.take<splat_adray_t>((size_t)R);
   w.keys = c.take<uint64_t>((size_t)R);
+  w.keys2 = c.take<uint64_t>((size_t)R);
   w.tile_counts = c.take<int32_t>((size_t)n_tiles);
   w.tile_offsets = c.take<int32_t>((size_t)n_tiles);
   w.counters = c.take<int64_t>(MAX_DEPTH + 2);
@@ -93,12 +96,12 @@ void hit_keys_init(int64_t R, const int64_t* d_count, const splat_hitset_t* hs, 
   if (hs->kind == SPLAT_HITTER_BV) launch_key_init(keys, R, d_count, s);  // brute force writes every key
 }
 void hit_keys_launch(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs,
-                     uint64_t* keys, hipStream_t s) {
+                     uint64_t* keys, hipStream_t s, const splat_pinhole_t* cam = nullptr) {
   if (hs->kind == SPLAT_HITTER_BF)
     launch_hit_bf(rays, R, d_count, static_cast<const TriRec*>(hs->d_recs), hs->n_tris, nullptr, keys, bf_waves(hs, R),
-                  hs->d_stats, s);
+                  hs->d_stats, s, cam);
   else
-    launch_hit_bv(rays, R, d_count, bv_scene(hs), keys, hs->d_stats, s);
+    launch_hit_bv(rays, R, d_count, bv_scene(hs), keys, hs->d_stats, s, cam);
 }
 
 // nearest hit index of the live rays into hit_idx (keys: scratch of the BV hitter)
@@ -154,47 +157,64 @@ int32_t check_lightmap(const splat_lightmap_params_t* lm) {
   return SPLAT_OK;
 }
 
-// the device-resident bounce loop over rays already in w.s1; results land in `final_rays`
+// the device-resident bounce loop; results land in `final_rays`. The live rays start in w.s1, or,
+// with cam != nullptr (and depth >= 1), are that pinhole's rays, generated where the first
+// bounce needs them. With lm != nullptr, rays that become DIFFUSE are final-evolved and splatted
+// into tex in the loop (the light map of the frame), otherwise they are left as run_evolution!
+// leaves them.
 int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris,
-                 int32_t* hit_final, const splat_run_params_t* p, const RunWs& w, hipStream_t s) {
+                 int32_t* hit_final, const splat_run_params_t* p, const RunWs& w, hipStream_t s,
+                 const splat_lightmap_params_t* lm = nullptr, float* tex = nullptr,
+                 const splat_pinhole_t* cam = nullptr) {
   const int64_t n_tiles = (R + TILE - 1) / TILE;
   const bool ev = p->hit_events && p->n_hit_events >= 2 * (p->depth + 1);
-  const bool ev_all = p->hit_events && p->n_hit_events >= 4 * p->depth + 4;
+  const bool ev_all = p->hit_events && p->n_hit_events >= 4 * p->depth + 2;
   auto mark = [&](int i) { if (ev) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[i]), s); };
   auto mark_part = [&](int i) {
     if (ev_all) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[2 * (p->depth + 1) + i]), s);
   };
+  const bool bv = hs->kind == SPLAT_HITTER_BV;
+  BounceParams P;
+  P.tris = tris;
+  P.n_tris = hs->n_tris;
+  P.first_diffuse = p->first_diffuse;
+  P.force_rand = p->force_rand;
+  P.seed = p->seed;
+  P.optics = p->optics;
+  P.tex = lm ? tex : nullptr;
   launch_set_i64(&w.counters[0], R, s);
   splat_adray_t* cur = w.s1;
   splat_adray_t* nxt = w.s2;
+  uint64_t* kc = w.keys;   // this bounce's keys
+  uint64_t* kn = w.keys2;  // the next bounce's, set to the miss key by the bounce pass
+  hit_keys_init(R, &w.counters[0], hs, kc, s);
   for (int k = 1; k <= p->depth; ++k) {
     const int64_t* live = &w.counters[k - 1];
-    hit_keys_init(R, live, hs, w.keys, s);
+    const splat_pinhole_t* gen = k == 1 ? cam : nullptr;
     mark(2 * (k - 1));
-    hit_keys_launch(cur, R, live, hs, w.keys, s);
+    hit_keys_launch(cur, R, live, hs, kc, s, gen);
     mark(2 * (k - 1) + 1);
-    launch_evolve(cur, R, live, nullptr, w.keys, tris, hs->n_tris, nullptr, p->force_rand, p->seed, k, p->first_diffuse, p->optics,
-                  w.tile_counts, s);
+    P.bounce = k;
+    launch_bounce_count(cur, R, live, kc, P, gen, w.tile_counts, s);
     launch_scan_tiles(w.tile_counts, n_tiles, live, R, w.tile_offsets, &w.counters[k], s);
     mark_part(2 * (k - 1));
-    launch_scatter(cur, R, live, w.tile_offsets, &w.counters[k], nxt, final_rays, nullptr, w.keys, hit_final, s);
+    launch_bounce(cur, R, live, kc, bv ? kn : nullptr, P, lm, gen, w.tile_offsets, &w.counters[k], nxt, final_rays,
+                  hit_final, s);
     mark_part(2 * (k - 1) + 1);
-    splat_adray_t* t = cur;
-    cur = nxt;
-    nxt = t;
+    std::swap(cur, nxt);
+    std::swap(kc, kn);
   }
   const int64_t* live = &w.counters[p->depth];
   // the reference's realigning next_hit! (src/tracer.jl:280)
-  if (hs->kind == SPLAT_HITTER_BF) {
+  if (!bv) {
     mark(2 * p->depth);
     hit_index(cur, R, live, hs, hit_final, nullptr, s);
     mark(2 * p->depth + 1);
   } else {
-    hit_keys_init(R, live, hs, w.keys, s);
     mark(2 * p->depth);
-    hit_keys_launch(cur, R, live, hs, w.keys, s);
+    hit_keys_launch(cur, R, live, hs, kc, s);  // kc holds the miss keys of the live rays
     mark(2 * p->depth + 1);
-    launch_key_final(w.keys, R, live, hit_final, s);
+    launch_key_final(kc, R, live, hit_final, s);
   }
   launch_copy_rays(cur, final_rays, R, live, s);
   return launch_status();
@@ -469,12 +489,10 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
   RunWs w = carve_run(ws, R);
   hipStream_t s = S(stream);
   launch_fill_f32(tex, (int64_t)lm->width * lm->height * lm->channels, 0.0f, s);
-  launch_raygen(*cam, w.s1, R, s);
-  if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s)) return e;
-  const bool ev_all = p->hit_events && p->n_hit_events >= 4 * p->depth + 4;
-  if (ev_all) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[4 * p->depth + 2]), s);
-  launch_light_map(rays, R, nullptr, hit_idx, tris, N, *lm, tex, s);
-  if (ev_all) (void)hipEventRecord(static_cast<hipEvent_t>(p->hit_events[4 * p->depth + 3]), s);
+  // the first bounce generates its rays itself; only a frame without bounces needs them stored
+  if (p->depth == 0) launch_raygen(*cam, w.s1, R, s);
+  // the light map is splatted inside the loop, as rays become DIFFUSE
+  if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s, lm, tex, p->depth > 0 ? cam : nullptr)) return e;
   return launch_status();
 }
 

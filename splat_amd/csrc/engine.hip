@@ -2,16 +2,17 @@
 //
 // Frames live in slots. A slot at stage s (bounces done) takes part in the next step as:
 //   s == 0      : clear its light map, generate its rays, then bounce 1
-//   0 <= s < d  : hit (live rays) -> evolve -> per-frame scan -> stable scatter   (bounce s+1)
-//   s == d      : realigning hit -> copy of the live rays -> final evolution + splat -> free
+//   0 <= s < d  : hit (live rays) -> count -> per-frame scan -> fused bounce pass (evolve, splat of
+//                 the rays that become DIFFUSE, stable partition; bounce.hpp)     (bounce s+1)
+//   s == d      : realigning hit -> copy of the live rays -> free
 // A step is one fixed sequence of launches for every slot at once: a planning kernel turns the
 // device-side live counts into per-kind work-item prefixes (and the hitter's split factor); the
 // hitter runs one wave per (64-ray group, split) item of any frame, so frames of any size and
 // scene cost share the chip; the other stages walk their items grid-stride. Launch shapes depend
 // only on the hitter's grid size class, so each class is captured into a HIP graph once and
-// replayed. The host tracks stages itself (every
-// admitted frame advances exactly one stage per step), so it never reads device state to learn
-// that a frame has finished: an event recorded after each step tells when.
+// replayed. The host tracks stages itself (every admitted frame advances exactly one stage per
+// step), so it never reads device state to learn that a frame has finished: an event recorded
+// after each step tells when.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,13 +23,13 @@
 
 #include "hitcore.hpp"
 #include "path.hpp"
-#include "scatter.hpp"
+#include "bounce.hpp"
 
 namespace spl {
 
 constexpr int MAX_SLOTS = 1024;
 constexpr int MAXD = SPLAT_ENGINE_MAX_DEPTH;
-enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, K_LMAP, N_KINDS };
+enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, N_KINDS };
 constexpr int HIT_ITEM = 64;     // rays per hitter work item (one wave)
 constexpr int CLEAR_ITEM = 4096;  // light-map floats per clear item
 
@@ -36,7 +37,7 @@ struct SlotDev {
   splat_adray_t* F;        // final rays (the frame's output order)
   splat_adray_t* S[2];     // live rays, ping-pong by stage parity
   int32_t* hit_final;
-  uint64_t* keys;          // packed (t, index) nearest hit of each live ray
+  uint64_t* keys[2];       // packed (t, index) nearest hit of each live ray, by stage parity
   int32_t* tile_counts;
   int32_t* tile_offsets;
   float* tex;
@@ -79,22 +80,20 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
   const int t = threadIdx.x;
   const bool in = t < E.n_slots;
   const int32_t s = in ? E.stage[t] : -1;
-  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0, 0};
+  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0};
   if (s >= 0) {
     const SlotDev& D = E.slots[t];
     int64_t* C = E.counters + (int64_t)t * (MAXD + 2);
     if (s == 0) C[0] = D.R;
     const int64_t A = s == 0 ? D.R : C[s];
     if (s == 0) {
-      cnt[K_RAYGEN] = (int32_t)cdiv(D.R, TILE);
+      // the first bounce generates its rays where it needs them; a frame without bounces stores them
+      cnt[K_RAYGEN] = D.depth == 0 ? (int32_t)cdiv(D.R, TILE) : 0;
       cnt[K_CLEAR] = (int32_t)cdiv(D.tex_n, CLEAR_ITEM);
     }
     cnt[K_HIT] = (int32_t)cdiv(A, HIT_ITEM);
     if (s < D.depth) cnt[K_TILE] = (int32_t)cdiv(A, TILE);
-    if (s == D.depth) {
-      cnt[K_FINAL] = (int32_t)cdiv(A, TILE);
-      cnt[K_LMAP] = (int32_t)cdiv(D.R, TILE);
-    }
+    if (s == D.depth) cnt[K_FINAL] = (int32_t)cdiv(A, TILE);
   }
   for (int k = 0; k < N_KINDS; ++k) {
     buf[t] = cnt[k];
@@ -129,16 +128,16 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
     const int64_t b = (int64_t)(i - Q[s]) * CLEAR_ITEM;
     for (int64_t k = b + threadIdx.x; k < b + CLEAR_ITEM && k < D.tex_n; k += blockDim.x) D.tex[k] = 0.0f;
   }
-  // keys of every live ray <- (Inf, 1) before the hitter merges into them
+  // keys of a new frame's rays <- (Inf, 1) before the hitter merges into them (later stages get
+  // theirs from the bounce pass)
   const int32_t* H = E.pre + K_HIT * (MAX_SLOTS + 1);
   constexpr int PER_BLOCK = 256 / HIT_ITEM;
   for (int32_t i = blockIdx.x * PER_BLOCK + threadIdx.x / HIT_ITEM; i < H[n]; i += gridDim.x * PER_BLOCK) {
     const int32_t s = item_slot(H, n, i);
+    if (E.stage[s] != 0) continue;
     const SlotDev& D = E.slots[s];
-    const int32_t st = E.stage[s];
-    const int64_t A = st == 0 ? D.R : E.counters[(int64_t)s * (MAXD + 2) + st];
     const int64_t r = (int64_t)(i - H[s]) * HIT_ITEM + (threadIdx.x & (HIT_ITEM - 1));
-    if (r < A) D.keys[r] = KEY_MISS;
+    if (r < D.R) D.keys[0][r] = KEY_MISS;
   }
 }
 
@@ -160,15 +159,39 @@ __global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
     const int32_t st = E.stage[s];
     const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
     const int64_t base = (int64_t)(item - P[s]) * HIT_ITEM;
-    switch (bv_mode(D.scene)) {
-      case BV_TREE: bv_item<false, BV_TREE>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs); break;
-      case BV_GROUPS: bv_item<false, BV_GROUPS>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs); break;
-      default: bv_item<false, BV_FLAT>(D.S[st & 1], A, base, i - item * S, S, D.scene, D.keys, hs); break;
+    uint64_t* keys = D.keys[st & 1];
+    const int32_t sp = i - item * S;
+    if (st == 0 && D.depth > 0) {  // the first bounce: the pinhole's rays, generated here
+      switch (bv_mode(D.scene)) {
+        case BV_TREE: bv_item<false, BV_TREE, true>(nullptr, A, base, sp, S, D.scene, keys, hs, &D.cam); break;
+        case BV_GROUPS: bv_item<false, BV_GROUPS, true>(nullptr, A, base, sp, S, D.scene, keys, hs, &D.cam); break;
+        default: bv_item<false, BV_FLAT, true>(nullptr, A, base, sp, S, D.scene, keys, hs, &D.cam); break;
+      }
+    } else {
+      switch (bv_mode(D.scene)) {
+        case BV_TREE: bv_item<false, BV_TREE>(D.S[st & 1], A, base, sp, S, D.scene, keys, hs); break;
+        case BV_GROUPS: bv_item<false, BV_GROUPS>(D.S[st & 1], A, base, sp, S, D.scene, keys, hs); break;
+        default: bv_item<false, BV_FLAT>(D.S[st & 1], A, base, sp, S, D.scene, keys, hs); break;
+      }
     }
   }
 }
 
-__global__ __launch_bounds__(TILE) void k_b_evolve(EngineDev E) {
+__device__ __forceinline__ BounceParams slot_bounce(const SlotDev& D, int32_t bounce) {
+  BounceParams P;
+  P.tris = D.tris;
+  P.n_tris = D.n_tris;
+  P.first_diffuse = D.first_diffuse;
+  P.bounce = bounce;
+  P.force_rand = D.force_rand;
+  P.seed = D.seed;
+  P.optics = D.optics;
+  P.tex = D.tex;
+  return P;
+}
+
+// count pass of the fused bounce (bounce.hpp): ACTIVE rays per tile from the hit keys
+__global__ __launch_bounds__(TILE) void k_b_count(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
   for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
@@ -177,20 +200,12 @@ __global__ __launch_bounds__(TILE) void k_b_evolve(EngineDev E) {
     const int32_t st = E.stage[s];
     const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
     const int32_t tile = i - P[s];
-    const int64_t r = (int64_t)tile * TILE + threadIdx.x;
-    splat_adray_t* rays = D.S[st & 1];
-    bool active_after = false;
-    if (r < A && rays[r].status == SPLAT_RAY_ACTIVE) {
-      RayF ray = load_rayf(rays + r);
-      int32_t h = key_index(D.keys[r]);
-      if (h < 1 || h > D.n_tris) h = 1;
-      const FTriF F = load_ftri(D.tris + (h - 1));
-      const float u = bounce_random(D.force_rand, nullptr, r, D.seed, st + 1, ray.dest);
-      RayF o = evolve_one(ray, h, F, u, D.first_diffuse, D.optics);
-      store_rayf(rays + r, o);
-      active_after = o.status == SPLAT_RAY_ACTIVE;
-    }
-    const int c = __syncthreads_count(active_after);
+    const int64_t base = (int64_t)tile * TILE;
+    const int nvalid = (int)(A - base < TILE ? A - base : TILE);
+    const BounceParams BP = slot_bounce(D, st + 1);
+    const uint64_t* keys = D.keys[st & 1] + base;
+    const int c = st == 0 ? bounce_count<true>(nullptr, keys, nvalid, BP, true, D.cam, base)
+                          : bounce_count<false>(D.S[st & 1] + base, keys, nvalid, BP, true, D.cam, base);
     if (threadIdx.x == 0) D.tile_counts[tile] = c;
   }
 }
@@ -226,7 +241,8 @@ __global__ __launch_bounds__(1024) void k_b_scan(EngineDev E) {
   if (t == 1023) C[st + 1] = sums[1023];
 }
 
-__global__ __launch_bounds__(TILE) void k_b_scatter(EngineDev E) {
+// bounce pass (bounce.hpp): evolve, splat the rays that become DIFFUSE, stable partition
+__global__ __launch_bounds__(TILE) void k_b_bounce(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
   for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
@@ -240,8 +256,15 @@ __global__ __launch_bounds__(TILE) void k_b_scatter(EngineDev E) {
     const int nvalid = (int)(A - base < TILE ? A - base : TILE);
     const int64_t off_a = D.tile_offsets[tile];
     const int64_t off_r = C[st + 1] + (base - off_a);
-    scatter_tile(D.S[st & 1] + base, nvalid, off_a, off_r, D.S[(st + 1) & 1], D.F, nullptr, D.hit_final,
-                 D.keys + base);
+    const BounceParams BP = slot_bounce(D, st + 1);
+    const uint64_t* keys = D.keys[st & 1] + base;
+    uint64_t* keys_next = D.keys[(st + 1) & 1];
+    if (st == 0)
+      bounce_tile<true>(nullptr, keys, keys_next, nvalid, BP, E.lm[s], D.cam, base, off_a, off_r, D.S[1], D.F,
+                        D.hit_final);
+    else
+      bounce_tile<false>(D.S[st & 1] + base, keys, keys_next, nvalid, BP, E.lm[s], D.cam, base, off_a, off_r,
+                         D.S[(st + 1) & 1], D.F, D.hit_final);
   }
 }
 
@@ -260,22 +283,7 @@ __global__ __launch_bounds__(TILE) void k_b_final(EngineDev E) {
     const uint2* src = reinterpret_cast<const uint2*>(D.S[st & 1] + base);
     uint2* dst = reinterpret_cast<uint2*>(D.F + base);
     for (int64_t k = threadIdx.x; k < nw; k += TILE) dst[k] = src[k];
-    if (base + threadIdx.x < A) D.hit_final[base + threadIdx.x] = key_index(D.keys[base + threadIdx.x]);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_b_lmap(EngineDev E) {
-  const int32_t n = E.n_slots;
-  const int32_t* P = E.pre + K_LMAP * (MAX_SLOTS + 1);
-  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
-    const int32_t s = item_slot(P, n, i);
-    const SlotDev& D = E.slots[s];
-    const int64_t r = (int64_t)(i - P[s]) * TILE + threadIdx.x;
-    if (r >= D.R || D.F[r].status != SPLAT_RAY_DIFFUSE) continue;
-    RayF ray = load_rayf(D.F + r);
-    int32_t h = D.hit_final[r];
-    if (h < 1 || h > D.n_tris) h = 1;
-    final_evolve_and_splat(ray, h, load_ftri(D.tris + (h - 1)), E.lm[s], D.tex, D.F + r);
+    if (base + threadIdx.x < A) D.hit_final[base + threadIdx.x] = key_index(D.keys[st & 1][base + threadIdx.x]);
   }
 }
 
@@ -346,7 +354,7 @@ struct Pending {
 
 size_t frame_bytes(int64_t R, bool with_outputs) {
   int64_t tiles = (R + TILE - 1) / TILE;
-  size_t b = 2 * al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 8) + 2 * al((size_t)tiles * 4);
+  size_t b = 2 * al((size_t)R * sizeof(splat_adray_t)) + 2 * al((size_t)R * 8) + 2 * al((size_t)tiles * 4);
   if (with_outputs) b += al((size_t)R * sizeof(splat_adray_t)) + al((size_t)R * 4);
   return b;
 }
@@ -380,11 +388,10 @@ int32_t launch_step_kernels(splat_engine* e, int64_t hit_blocks, hipStream_t s) 
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
   hipLaunchKernelGGL(k_b_raygen, dim3(e->grid), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
-  hipLaunchKernelGGL(k_b_evolve, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_count, dim3(e->grid), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
-  hipLaunchKernelGGL(k_b_scatter, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_bounce, dim3(e->grid), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_final, dim3(e->grid), dim3(TILE), 0, s, E);
-  hipLaunchKernelGGL(k_b_lmap, dim3(e->grid), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_finish, dim3(1), dim3(1024), 0, s, E);
   return hipGetLastError() == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
 }
@@ -483,7 +490,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   if (slot < 0) return SPLAT_ERR_BUDGET;
   HostSlot& hsl = e->slots[slot];
   const int64_t tiles = (R + TILE - 1) / TILE;
-  std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 8, tiles * 4, tiles * 4};
+  std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 8, tiles * 4, tiles * 4, R * 8};
   if (!rq->rays_out) sizes.push_back(R * (int64_t)sizeof(splat_adray_t));
   if (!rq->hit_out) sizes.push_back(R * 4);
   std::vector<Block> got;
@@ -500,10 +507,11 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   std::memset(&D, 0, sizeof(D));
   D.S[0] = reinterpret_cast<splat_adray_t*>(at(0));
   D.S[1] = reinterpret_cast<splat_adray_t*>(at(1));
-  D.keys = reinterpret_cast<uint64_t*>(at(2));
+  D.keys[0] = reinterpret_cast<uint64_t*>(at(2));
   D.tile_counts = reinterpret_cast<int32_t*>(at(3));
   D.tile_offsets = reinterpret_cast<int32_t*>(at(4));
-  size_t k = 5;
+  D.keys[1] = reinterpret_cast<uint64_t*>(at(5));
+  size_t k = 6;
   D.F = rq->rays_out ? rq->rays_out : reinterpret_cast<splat_adray_t*>(at(k++));
   D.hit_final = rq->hit_out ? rq->hit_out : reinterpret_cast<int32_t*>(at(k++));
   D.tex = rq->tex;

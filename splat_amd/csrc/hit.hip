@@ -11,6 +11,8 @@
 //    global atomics, ties to the lowest index;
 //  * the BV hitter walks the conservative member boxes per wave; a box whose slab test fails for
 //    all 64 lanes (one ballot) is skipped, otherwise its member records are scanned.
+#include <cstring>
+
 #include "hitcore.hpp"
 
 namespace spl {
@@ -92,7 +94,8 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
                                                    const int64_t* __restrict__ d_count,
                                                    const TriRec* __restrict__ recs, int32_t N,
                                                    int32_t* __restrict__ hit_idx, uint64_t* __restrict__ keys,
-                                                   unsigned long long* stats) {
+                                                   unsigned long long* stats, const splat_pinhole_t cam,
+                                                   int32_t from_cam) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t Rl = live_count(R, d_count);
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
   if (base >= Rl) return;  // uniform over the workgroup
   const int64_t r = base + lane;
   const bool valid = r < Rl;
-  RayLanes L = load_ray_lanes(rays, r, valid);
+  RayLanes L = from_cam ? cam_ray_lanes(cam, r, valid) : load_ray_lanes(rays, r, valid);
   float best = INFINITY;
   int32_t bi = 1;
 #pragma unroll 4
@@ -152,7 +155,8 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
 template <bool STATS, int MODE>
 __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
                                                 const int64_t* __restrict__ d_count, BVScene scene,
-                                                uint64_t* __restrict__ keys, unsigned long long* stats) {
+                                                uint64_t* __restrict__ keys, unsigned long long* stats,
+                                                const splat_pinhole_t cam, int32_t from_cam) {
   const int64_t A = live_count(R, d_count);
   const int64_t groups64 = (A + 63) / 64;
   const int32_t units = MODE == BV_TREE ? (1 << scene.levels) : MODE == BV_GROUPS ? scene.n_groups : scene.n_bv;
@@ -161,7 +165,8 @@ __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict_
   if (i >= groups64 * S) return;  // whole waves
   HitStats st;
   const int64_t g = i / S;
-  bv_item<STATS, MODE>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st);
+  if (from_cam) bv_item<STATS, MODE, true>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st, &cam);
+  else bv_item<STATS, MODE, false>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st);
   if constexpr (STATS) {
     atomicAdd(stats + 0, st.n_box);
     atomicAdd(stats + 1, st.n_enter);
@@ -279,23 +284,31 @@ int choose_bf_waves(int64_t R, int32_t N) {
 
 template <bool ST>
 static void launch_hit_bf_t(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
-                            int32_t* hit_idx, uint64_t* keys, int waves, unsigned long long* st, hipStream_t s) {
+                            int32_t* hit_idx, uint64_t* keys, int waves, unsigned long long* st,
+                            const splat_pinhole_t& c, int32_t fc, hipStream_t s) {
   dim3 grid((unsigned)((R + 63) / 64));
+#define SPL_HIT_BF(W) \
+  hipLaunchKernelGGL((k_hit_bf<W, ST>), grid, dim3(64 * W), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st, c, fc)
   switch (waves) {
-    case 1: hipLaunchKernelGGL((k_hit_bf<1, ST>), grid, dim3(64), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
-    case 2: hipLaunchKernelGGL((k_hit_bf<2, ST>), grid, dim3(128), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
-    case 4: hipLaunchKernelGGL((k_hit_bf<4, ST>), grid, dim3(256), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
-    case 8: hipLaunchKernelGGL((k_hit_bf<8, ST>), grid, dim3(512), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
-    default: hipLaunchKernelGGL((k_hit_bf<16, ST>), grid, dim3(1024), 0, s, rays, R, d_count, recs, N, hit_idx, keys, st); break;
+    case 1: SPL_HIT_BF(1); break;
+    case 2: SPL_HIT_BF(2); break;
+    case 4: SPL_HIT_BF(4); break;
+    case 8: SPL_HIT_BF(8); break;
+    default: SPL_HIT_BF(16); break;
   }
+#undef SPL_HIT_BF
 }
 
 void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
-                   int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s) {
+                   int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s,
+                   const splat_pinhole_t* cam) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
-  if (st) launch_hit_bf_t<true>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, s);
-  else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, s);
+  splat_pinhole_t c;
+  if (cam) c = *cam;
+  else std::memset(&c, 0, sizeof(c));
+  if (st) launch_hit_bf_t<true>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, c, cam ? 1 : 0, s);
+  else launch_hit_bf_t<false>(rays, R, d_count, recs, N, hit_idx, keys, waves, st, c, cam ? 1 : 0, s);
 }
 
 void launch_tree_build(const float4* boxes, const int2* ranges, int32_t n_bv, int32_t levels, float4* nodes,
@@ -327,11 +340,16 @@ void launch_key_final(const uint64_t* keys, int64_t R, const int64_t* d_count, i
 
 // keys[] must have been initialised (launch_key_init) on the same stream
 void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const BVScene& scene, uint64_t* keys,
-                   uint64_t* stats, hipStream_t s) {
+                   uint64_t* stats, hipStream_t s, const splat_pinhole_t* cam) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
   dim3 grid((unsigned)((max_hit_items((R + 63) / 64) + 3) / 4)), block(256);
-#define SPL_HIT_BV(ST, M) hipLaunchKernelGGL((k_hit_bv<ST, M>), grid, block, 0, s, rays, R, d_count, scene, keys, st)
+  splat_pinhole_t c;
+  if (cam) c = *cam;
+  else std::memset(&c, 0, sizeof(c));
+  const int32_t fc = cam ? 1 : 0;
+#define SPL_HIT_BV(ST, M) \
+  hipLaunchKernelGGL((k_hit_bv<ST, M>), grid, block, 0, s, rays, R, d_count, scene, keys, st, c, fc)
   switch (bv_mode(scene)) {
     case BV_TREE: if (st) SPL_HIT_BV(true, BV_TREE); else SPL_HIT_BV(false, BV_TREE); break;
     case BV_GROUPS: if (st) SPL_HIT_BV(true, BV_GROUPS); else SPL_HIT_BV(false, BV_GROUPS); break;

@@ -2,7 +2,7 @@
 // (hit.hip) and the multi-frame engine (engine.hip).
 #pragma once
 
-#include "internal.hpp"
+#include "path.hpp"
 
 namespace spl {
 
@@ -20,6 +20,24 @@ __device__ __forceinline__ RayLanes load_ray_lanes(const splat_adray_t* __restri
     L.ig = p->ignore_tri;
   }
   return L;
+}
+
+// the lanes of pinhole ray r, generated on the fly (the value parts of camera_ray)
+__device__ __forceinline__ RayLanes cam_ray_lanes(const splat_pinhole_t& cam, int64_t r, bool valid) {
+  RayLanes L{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+  if (valid) {
+    const splat_adray_t g = raygen_ray(cam, r);
+    L.ox = g.pos[0]; L.oy = g.pos[1]; L.oz = g.pos[2];
+    L.dx = g.dir[0]; L.dy = g.dir[1]; L.dz = g.dir[2];
+    L.ig = g.ignore_tri;
+  }
+  return L;
+}
+template <bool CAM>
+__device__ __forceinline__ RayLanes ray_lanes(const splat_adray_t* __restrict__ rays, const splat_pinhole_t& cam,
+                                              int64_t r, bool valid) {
+  if constexpr (CAM) return cam_ray_lanes(cam, r, valid);
+  else return load_ray_lanes(rays, r, valid);
 }
 
 __device__ __forceinline__ int64_t live_count(int64_t R, const int64_t* __restrict__ d_count) {
@@ -229,16 +247,16 @@ inline __host__ __device__ int64_t max_hit_items(int64_t groups64) {
   return groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS;
 }
 
-// One (64-ray group, split) work item: rays [base, base + 64) of `rays` with A live, scanning
-// split `split` of `n_split`; the result is merged into keys[r] with a 64-bit atomic minimum of
-// the packed (t, index) key. Called by whole waves.
-template <bool STATS, int MODE>
+// One (64-ray group, split) work item: rays [base, base + 64) of `rays` (CAM: of the pinhole) with
+// A live, scanning split `split` of `n_split`; the result is merged into keys[r] with a 64-bit
+// atomic minimum of the packed (t, index) key. Called by whole waves.
+template <bool STATS, int MODE, bool CAM = false>
 __device__ __forceinline__ void bv_item(const splat_adray_t* __restrict__ rays, int64_t A, int64_t base, int32_t split,
                                         int32_t n_split, const BVScene& scene, uint64_t* __restrict__ keys,
-                                        HitStats& st) {
+                                        HitStats& st, const splat_pinhole_t* cam = nullptr) {
   const int64_t r = base + (threadIdx.x & 63);
   const bool valid = r < A;
-  const RayLanes L = load_ray_lanes(rays, r, valid);
+  const RayLanes L = CAM ? cam_ray_lanes(*cam, r, valid) : load_ray_lanes(rays, r, valid);
   float best;
   const int32_t bi = bv_traverse<STATS, MODE>(L, valid, scene, st, &best, split, n_split,
                                                 n_split > 1 && valid ? keys + r : nullptr);

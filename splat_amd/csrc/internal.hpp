@@ -15,8 +15,10 @@ void launch_bv_boxes(const splat_ftri_t* tris, const int32_t* members, const int
                      hipStream_t s);
 int choose_bf_waves(int64_t R, int32_t N);
 // brute force: writes hit_idx, or the packed (t, index) keys when keys != nullptr
+// cam != nullptr (hitters): the rays are that pinhole's rays 0..R-1, generated on the fly
 void launch_hit_bf(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const TriRec* recs, int32_t N,
-                   int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s);
+                   int32_t* hit_idx, uint64_t* keys, int waves, uint64_t* stats, hipStream_t s,
+                   const splat_pinhole_t* cam = nullptr);
 void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, float4* groups, hipStream_t s);
 // Binary tree over the BVs in their order, heap-indexed: node n has children 2n and 2n + 1, the
 // 2^levels leaves are nodes 2^levels + b (BV b, or empty padding). nodes[2n] / nodes[2n + 1] are
@@ -63,10 +65,19 @@ inline __host__ __device__ int bv_mode(const BVScene& s) {
 // BV hitter into packed keys (initialised by launch_key_init on the same stream)
 void launch_key_init(uint64_t* keys, int64_t R, const int64_t* d_count, hipStream_t s);
 void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const BVScene& scene, uint64_t* keys,
-                   uint64_t* stats, hipStream_t s);
+                   uint64_t* stats, hipStream_t s, const splat_pinhole_t* cam = nullptr);
 void launch_key_final(const uint64_t* keys, int64_t R, const int64_t* d_count, int32_t* hit_idx, hipStream_t s);
 
 // trace.hip
+struct BounceParams;  // bounce.hpp
+// cam != nullptr: the live rays are that pinhole's rays, generated on the fly (first bounce)
+void launch_bounce_count(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
+                         const BounceParams& P, const splat_pinhole_t* cam, int32_t* tile_counts, hipStream_t s);
+// lm == nullptr: no splat (P.tex ignored); keys_next (nullable): miss keys at the next positions
+void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
+                   uint64_t* keys_next, const BounceParams& P, const splat_lightmap_params_t* lm,
+                   const splat_pinhole_t* cam, const int32_t* tile_offsets, const int64_t* d_total,
+                   splat_adray_t* out_active, splat_adray_t* out_rest, int32_t* hit_rest, hipStream_t s);
 // hit index per ray from hit_idx, or from the packed keys when keys != nullptr
 void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const uint64_t* keys,
                    const splat_ftri_t* tris, int32_t N, const float* rndm, float force_rand, uint64_t seed, int32_t bounce,

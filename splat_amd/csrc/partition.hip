@@ -28,9 +28,17 @@ __global__ __launch_bounds__(TILE) void k_count_active(const splat_adray_t* __re
 }
 
 // single workgroup, 1024 threads: each thread owns a contiguous chunk of tiles
+// exclusive scan of the tile counts of the live tiles (the first ceil(min(*d_count, R) / TILE)
+// of n_tiles; all of them without d_count)
 __global__ __launch_bounds__(1024) void k_scan_tiles(const int32_t* __restrict__ tile_counts, int64_t n_tiles,
+                                                     const int64_t* __restrict__ d_count, int64_t R,
                                                      int32_t* __restrict__ tile_offsets, int64_t* __restrict__ d_total) {
   __shared__ int64_t sums[1024];
+  if (d_count) {
+    const int64_t A = *d_count < R ? *d_count : R;
+    const int64_t live_tiles = (A + TILE - 1) / TILE;
+    n_tiles = live_tiles < n_tiles ? live_tiles : n_tiles;
+  }
   const int t = threadIdx.x;
   const int64_t chunk = (n_tiles + 1023) / 1024;
   const int64_t b = t * chunk, e = b + chunk < n_tiles ? b + chunk : n_tiles;
@@ -79,9 +87,7 @@ void launch_count_active(const splat_adray_t* rays, int64_t R, const int64_t* d_
 }
 void launch_scan_tiles(const int32_t* tile_counts, int64_t n_tiles, const int64_t* d_count, int64_t R,
                        int32_t* tile_offsets, int64_t* d_total, hipStream_t s) {
-  (void)d_count;
-  (void)R;
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_counts, n_tiles, tile_offsets, d_total);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tile_counts, n_tiles, d_count, R, tile_offsets, d_total);
 }
 void launch_scatter(const splat_adray_t* in, int64_t R, const int64_t* d_count, const int32_t* tile_offsets,
                     const int64_t* d_total, splat_adray_t* out_active, splat_adray_t* out_rest, const int32_t* carry_in,

@@ -71,6 +71,34 @@ __device__ __forceinline__ bool get_hit_ad(const RayF& r, const FTriF& F, int32_
   return true;
 }
 
+// The status evolve_one gives a live ray whose nearest hit (from the hitter) is i, known before
+// the evolve math: for i != 1 evolve's own hit test on triangle i repeats the hitter's winning
+// test (same record, same ray) and succeeds; for i == 1 it tests the sentinel triangle, which
+// only matters when first_diffuse > 1 (tri1_hit: that test, done by the caller only then).
+__device__ __forceinline__ uint8_t status_after(int32_t i, int32_t first_diffuse, bool tri1_hit) {
+  if (i != 1) return i >= first_diffuse ? SPLAT_RAY_DIFFUSE : SPLAT_RAY_ACTIVE;
+  if (1 >= first_diffuse) return SPLAT_RAY_DIFFUSE;
+  return tri1_hit ? SPLAT_RAY_ACTIVE : SPLAT_RAY_INFINITY;
+}
+// evolve's hit test of the sentinel triangle (tris[0]) for a ray
+__device__ __forceinline__ bool sentinel_hit(const splat_ftri_t* __restrict__ tris, const V3<float>& pos,
+                                             const V3<float>& dir, int64_t ignore_tri) {
+  const FTriF F1 = load_ftri(tris);
+  const TriRec P = tri_prep(F1.v[0], F1.v[1], F1.v[2], F1.v[3], 1);
+  float t;
+  return hit_rec(P.q0, P.q1, P.q2, P.q3, pos.x, pos.y, pos.z, dir.x, dir.y, dir.z, ignore_tri, t);
+}
+// Whether the sentinel's test can succeed at all: with a zero normal it computes t = 0/0 (NaN)
+// for every ray, so it never does (the usual, degenerate sentinel) and no ray need be read.
+__device__ __forceinline__ bool sentinel_live(const splat_ftri_t* __restrict__ tris) {
+  return tris[0].v[0][0] != 0.0f || tris[0].v[0][1] != 0.0f || tris[0].v[0][2] != 0.0f;
+}
+__device__ __forceinline__ uint8_t predict_status(int32_t i, int32_t first_diffuse, const splat_ftri_t* __restrict__ tris,
+                                                  const V3<float>& pos, const V3<float>& dir, int64_t ignore_tri) {
+  const bool need = i == 1 && first_diffuse > 1 && sentinel_live(tris);
+  return status_after(i, first_diffuse, need && sentinel_hit(tris, pos, dir, ignore_tri));
+}
+
 // evolve_ray + handle_optics (src/tracer.jl:181-201, :118-179)
 __device__ __forceinline__ RayF evolve_one(const RayF& r, int32_t i, const FTriF& F, float rndm, int32_t first_diffuse,
                                            const splat_optics_t& opt) {
@@ -140,11 +168,10 @@ __device__ __forceinline__ float curve_eval(const splat_lightmap_params_t& lm, i
   return FMA(slope, lam - x1, y1);
 }
 
-// final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, written back to ray_out, then
-// its splats (src/ray_imagers.jl:78-113) in the reference's thread order (k, dy, dx)
-__device__ __forceinline__ void final_evolve_and_splat(RayF r, int32_t i, const FTriF& F,
-                                                       const splat_lightmap_params_t& lm, float* __restrict__ tex,
-                                                       splat_adray_t* ray_out) {
+// final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, in place, then its splats
+// (src/ray_imagers.jl:78-113) in the reference's thread order (k, dy, dx)
+__device__ __forceinline__ void final_evolve_splat(RayF& r, int32_t i, const FTriF& F, const splat_lightmap_params_t& lm,
+                                                   float* __restrict__ tex) {
   // final_evolution
   D3<float> tD;
   bool hit = get_hit_ad(r, F, i, &tD);
@@ -156,7 +183,6 @@ __device__ __forceinline__ void final_evolve_and_splat(RayF r, int32_t i, const 
     r.dp[0] = r.dp[1] = r.dp[2] = r.pos;
   }
   r.ignore_tri = hit ? i : 1;
-  store_rayf(ray_out, r);
   // splat (k, dy, dx) in the reference's thread order
   const int nl = lm.n_lambda > 0 ? lm.n_lambda : 1;
   const float W = (float)lm.width, H = (float)lm.height;
@@ -197,9 +223,17 @@ __device__ __forceinline__ void final_evolve_and_splat(RayF r, int32_t i, const 
   }
 }
 
+// final evolution + splat of one DIFFUSE ray, written back to ray_out
+__device__ __forceinline__ void final_evolve_and_splat(RayF r, int32_t i, const FTriF& F,
+                                                       const splat_lightmap_params_t& lm, float* __restrict__ tex,
+                                                       splat_adray_t* ray_out) {
+  final_evolve_splat(r, i, F, lm, tex);
+  store_rayf(ray_out, r);
+}
+
 // camera_ray (src/ray_generators.jl:18-50) for ray q of a pinhole frame, evaluated in Float64 and
 // rounded to the Float32 ADRay fields as the reference does
-__device__ __forceinline__ void raygen_one(const splat_pinhole_t& cam, int64_t q, splat_adray_t* out) {
+__device__ __forceinline__ splat_adray_t raygen_ray(const splat_pinhole_t& cam, int64_t q) {
 
   const int64_t H = cam.height, Wd = cam.width, L = cam.samples_per_pixel;
   const int64_t P = q / L, l = q % L;
@@ -250,7 +284,10 @@ __device__ __forceinline__ void raygen_one(const splat_pinhole_t& cam, int64_t q
   o.y = 0.f;
   o.status = SPLAT_RAY_ACTIVE;
   o._pad1[0] = o._pad1[1] = o._pad1[2] = 0;
-  *out = o;
+  return o;
+}
+__device__ __forceinline__ void raygen_one(const splat_pinhole_t& cam, int64_t q, splat_adray_t* out) {
+  *out = raygen_ray(cam, q);
 }
 
 }  // namespace spl

@@ -1,7 +1,49 @@
-// trace.hip -- dual-number evolve, final evolution + light-map splat, pinhole ray source.
-#include "path.hpp"
+// trace.hip -- dual-number evolve, final evolution + light-map splat, pinhole ray source, and the
+// fused count / bounce passes of the bounce loop.
+#include <cstring>
+
+#include "bounce.hpp"
 
 namespace spl {
+
+// ---- fused bounce loop passes (bounce.hpp) ----------------------------------------------------
+// from_cam: the live rays are the pinhole's rays 0..R-1, not yet generated (the first bounce)
+__global__ __launch_bounds__(TILE) void k_bounce_count(const splat_adray_t* __restrict__ rays, int64_t R,
+                                                       const int64_t* __restrict__ d_count,
+                                                       const uint64_t* __restrict__ keys, BounceParams P,
+                                                       const splat_pinhole_t cam, int32_t from_cam,
+                                                       int32_t* __restrict__ tile_counts) {
+  const int64_t A = live_count_t(R, d_count);
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  if (base >= A) return;  // uniform over the workgroup
+  const int nvalid = (int)(A - base < TILE ? A - base : TILE);
+  const int c = from_cam ? bounce_count<true>(rays + base, keys + base, nvalid, P, true, cam, base)
+                         : bounce_count<false>(rays + base, keys + base, nvalid, P, true, cam, base);
+  if (threadIdx.x == 0) tile_counts[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(TILE) void k_bounce(const splat_adray_t* __restrict__ rays, int64_t R,
+                                                 const int64_t* __restrict__ d_count,
+                                                 const uint64_t* __restrict__ keys, uint64_t* __restrict__ keys_next,
+                                                 BounceParams P, const splat_lightmap_params_t lm,
+                                                 const splat_pinhole_t cam, int32_t from_cam,
+                                                 const int32_t* __restrict__ tile_offsets,
+                                                 const int64_t* __restrict__ d_total,
+                                                 splat_adray_t* __restrict__ out_active,
+                                                 splat_adray_t* __restrict__ out_rest, int32_t* __restrict__ hit_rest) {
+  const int64_t A = live_count_t(R, d_count);
+  const int64_t base = (int64_t)blockIdx.x * TILE;
+  if (base >= A) return;
+  const int nvalid = (int)(A - base < TILE ? A - base : TILE);
+  const int64_t off_a = tile_offsets[blockIdx.x];
+  const int64_t off_r = *d_total + (base - off_a);
+  if (from_cam)
+    bounce_tile<true>(rays + base, keys + base, keys_next, nvalid, P, lm, cam, base, off_a, off_r, out_active, out_rest,
+                      hit_rest);
+  else
+    bounce_tile<false>(rays + base, keys + base, keys_next, nvalid, P, lm, cam, base, off_a, off_r, out_active, out_rest,
+                       hit_rest);
+}
 
 __global__ __launch_bounds__(TILE) void k_evolve(splat_adray_t* __restrict__ rays, int64_t R,
                                                  const int64_t* __restrict__ d_count,
@@ -63,6 +105,32 @@ __global__ void k_set_i64(int64_t* dst, int64_t v) {
 __global__ void k_copy_rays(const uint2* __restrict__ src, uint2* __restrict__ dst, int64_t R, const int64_t* __restrict__ d_count) {
   const int64_t n = live_count_t(R, d_count) * (int64_t)(sizeof(splat_adray_t) / 8);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+void launch_bounce_count(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
+                         const BounceParams& P, const splat_pinhole_t* cam, int32_t* tile_counts, hipStream_t s) {
+  if (R <= 0) return;
+  splat_pinhole_t c;
+  if (cam) c = *cam;
+  else std::memset(&c, 0, sizeof(c));
+  hipLaunchKernelGGL(k_bounce_count, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, keys,
+                     P, c, cam ? 1 : 0, tile_counts);
+}
+void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
+                   uint64_t* keys_next, const BounceParams& P, const splat_lightmap_params_t* lm,
+                   const splat_pinhole_t* cam, const int32_t* tile_offsets, const int64_t* d_total,
+                   splat_adray_t* out_active, splat_adray_t* out_rest, int32_t* hit_rest, hipStream_t s) {
+  if (R <= 0) return;
+  splat_lightmap_params_t l;
+  if (lm) l = *lm;
+  else std::memset(&l, 0, sizeof(l));
+  splat_pinhole_t c;
+  if (cam) c = *cam;
+  else std::memset(&c, 0, sizeof(c));
+  BounceParams Q = P;
+  if (!lm) Q.tex = nullptr;
+  hipLaunchKernelGGL(k_bounce, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, keys,
+                     keys_next, Q, l, c, cam ? 1 : 0, tile_offsets, d_total, out_active, out_rest, hit_rest);
 }
 
 void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const uint64_t* keys,
