@@ -125,7 +125,9 @@ typedef struct splat_hitset {
                          /* or the binary tree's node boxes when tree_levels > 0             */
   int32_t group_size;    /* BVs per group; 0 = no coarse level (BV only)                  */
   int32_t n_groups;      /* ceil(n_bv / group_size)                                        */
-  int32_t tune_waves;    /* triangle-split waves per workgroup (0 = autotuned default)   */
+  int32_t tune_waves;    /* BF: triangle-split waves per workgroup (0 = autotuned default); */
+                         /* BV tree: 0 = waves walk together, 1 = per wave, together when */
+                         /* its rays share a direction octant, else per lane; 2 = per lane */
   int32_t tree_levels;   /* 1 + levels of the tree over the BVs; 0 = no tree (BV only)    */
   /* Optional device uint64[4] counters, accumulated by every hitter launch when non-NULL:
    * [0] ray-box slab tests, [1] (ray, box) pairs entered, [2] ray-triangle tests of the

@@ -83,7 +83,8 @@ __device__ __forceinline__ float key_t(uint64_t k) {
 //   BV_FLAT   -- every leaf box in order (the reference's one-level DPBVHitter);
 //   BV_GROUPS -- group boxes of gsize consecutive leaves, then their leaves;
 //   BV_TREE   -- depth-first over the binary tree, nearer child first along the node's split axis
-//                for the wave's first live ray, without a stack (heap indices: up = n / 2).
+//                for the wave's first live ray, without a stack (heap indices: up = n / 2); a
+//                wave whose rays diverge in direction may walk per lane (tree_walk_lanes).
 // Work split `split` of `n_split` takes groups/leaves split, split + n_split, ... (flat, groups)
 // or the subtree under node n_split + split (tree; n_split a power of two). Must be called by
 // whole waves.
@@ -100,6 +101,96 @@ using cptr = const T*;
 template <class T>
 __device__ __forceinline__ cptr<T> as_const(const T* p) {
   return (cptr<T>)(p);
+}
+
+// Per-lane variant of the tree walk, for waves whose rays diverge: every lane walks its own
+// depth-first path through the subtree under `root` (nearer child first along its own
+// direction), stackless -- a per-lane bit per level records whether the node on the path is its
+// parent's first child, so climbing needs no memory. Node boxes and member records come as
+// per-lane vector loads (cached: the tree and records are small and shared). Lanes first walk
+// until they reach a leaf they enter (or finish); then the lanes holding a leaf test its members
+// together; repeat until every lane is done. A lane's work is its own path, not the union of the
+// wave's paths. best/bi are in-out; shared_key as in bv_traverse.
+template <bool STATS>
+__device__ __forceinline__ void tree_walk_lanes(const RayLanes& L, bool valid, float ix, float iy, float iz,
+                                                const BVScene& S, int32_t root, HitStats& st, float& best,
+                                                int32_t& bi, const uint64_t* shared_key) {
+  const float4* __restrict__ nodes = S.nodes;
+  const int2* __restrict__ ranges = S.ranges;
+  const TriRec* __restrict__ recs = S.recs;
+  const int32_t leaf0 = 1 << S.levels;
+  const int32_t neg = (L.dx < 0.f ? 1 : 0) | (L.dy < 0.f ? 2 : 0) | (L.dz < 0.f ? 4 : 0);
+  int32_t n = root;
+  uint32_t firsts = 0;  // bit l: the node at depth l below root is its parent's first child
+  int32_t depth = 0;
+  bool done = !valid;
+  // advance past node n (its subtree finished or skipped)
+  auto next = [&]() {
+    while (depth > 0 && !((firsts >> depth) & 1u)) {
+      n >>= 1;
+      --depth;
+    }
+    if (depth == 0) {
+      done = true;
+    } else {
+      firsts &= ~(1u << depth);
+      n ^= 1;
+    }
+  };
+  while (__builtin_amdgcn_ballot_w64(!done) != 0) {
+    // walk until this lane holds a leaf it enters, or is done
+    int32_t leaf = -1;
+    while (__builtin_amdgcn_ballot_w64(!done && leaf < 0) != 0) {
+      if (!done && leaf < 0) {
+        const float4 lo = nodes[2 * n], hi = nodes[2 * n + 1];
+        const int32_t code = __float_as_int(lo.w);
+        const bool enter = !(code & TREE_EMPTY) && slab_enter(lo, hi, L, ix, iy, iz, best);
+        if constexpr (STATS) st.n_box += 1ull;
+        if (enter && n < leaf0) {
+          n = 2 * n + (((neg >> (code & TREE_AXIS)) & 1) ^ ((code & TREE_SWAP) ? 1 : 0));
+          ++depth;
+          firsts |= 1u << depth;
+        } else if (enter) {
+          leaf = n - leaf0;
+        } else {
+          next();
+        }
+      }
+    }
+    // the lanes holding a leaf test its members
+    int2 rg = make_int2(0, 0);
+    if (leaf >= 0) {
+      rg = ranges[leaf];
+      if (shared_key) {
+        const uint64_t k = __hip_atomic_load(shared_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k < pack_key(best, bi)) {
+          best = key_t(k);
+          bi = key_index(k);
+        }
+      }
+      if constexpr (STATS) {
+        st.n_enter += 1ull;
+        st.n_alg += (unsigned long long)rg.y;
+      }
+    }
+    int32_t m_max = rg.y;
+    for (int off = 32; off > 0; off >>= 1) m_max = max(m_max, __shfl_xor(m_max, off));
+    if constexpr (STATS) st.n_exec += (threadIdx.x & 63) == 0 ? (unsigned long long)m_max : 0ull;
+    for (int32_t m = 0; m < m_max; ++m) {
+      if (m < rg.y) {
+        const TriRec& T = recs[rg.x + m];
+        const float4 q0 = T.q0, q1 = T.q1, q2 = T.q2, q3 = T.q3;
+        float t;
+        const bool h = hit_rec(q0, q1, q2, q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
+        const int32_t idx = __float_as_int(q1.w);
+        if (h && (t < best || (t == best && idx < bi))) {
+          best = t;
+          bi = idx;
+        }
+      }
+    }
+    if (leaf >= 0) next();
+  }
 }
 
 template <bool STATS, int MODE>
@@ -177,48 +268,83 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     // splits beyond the leaf count have no subtree of their own
     const int32_t n_sub = n_split < leaf0 ? n_split : leaf0;
     const uint64_t live = __builtin_amdgcn_ballot_w64(valid);
-    if (live != 0 && split < n_sub) {
-      // direction signs of the wave's first live ray order the children
-      const int32_t fl = (int32_t)__builtin_ctzll(live);
-      const float rdx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dx), fl));
-      const float rdy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dy), fl));
-      const float rdz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dz), fl));
-      const int32_t neg = (rdx < 0.f ? 1 : 0) | (rdy < 0.f ? 2 : 0) | (rdz < 0.f ? 4 : 0);
-      auto first_child = [&](int32_t p, int32_t code) {
-        return 2 * p + (((neg >> (code & TREE_AXIS)) & 1) ^ ((code & TREE_SWAP) ? 1 : 0));
-      };
-      // the subtree of this split (n_split is a power of two)
+    // direction signs of the wave's first live ray order the children
+    const int32_t fl = live ? (int32_t)__builtin_ctzll(live) : 0;
+    const float rdx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dx), fl));
+    const float rdy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dy), fl));
+    const float rdz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L.dz), fl));
+    const int32_t neg = (rdx < 0.f ? 1 : 0) | (rdy < 0.f ? 2 : 0) | (rdz < 0.f ? 4 : 0);
+    // a wave whose live rays do not all point into the leader's octant walks per lane
+    const int32_t own = (L.dx < 0.f ? 1 : 0) | (L.dy < 0.f ? 2 : 0) | (L.dz < 0.f ? 4 : 0);
+    const uint64_t same = __builtin_amdgcn_ballot_w64(valid && own == neg);
+    const bool per_lane = S.walk == BV_WALK_LANES || (S.walk == BV_WALK_AUTO && same != live);
+    if (live != 0 && split < n_sub && per_lane) {
+      tree_walk_lanes<STATS>(L, valid, ix, iy, iz, S, n_sub + split, st, best, bi, shared_key);
+    } else if (live != 0 && split < n_sub) {
+      // The wave walks one path: a node's two children are loaded together (one 64-byte scalar
+      // load) and both tested; it goes into the nearer entered child and marks the farther one
+      // pending when entered too; a pending sibling is re-tested (with the nearer hits found
+      // meanwhile) when the walk comes back to it. Bit d of `firsts` / `pending`: the node on the
+      // path at depth d below the split's root is the nearer child / its sibling is pending, so
+      // climbing needs no memory.
       const int32_t root = n_sub + split;
-      int32_t n = root;
-      for (;;) {
-        const float4 lo = nodes[2 * n], hi = nodes[2 * n + 1];
-        const int32_t code = __float_as_int(lo.w);
-        bool descend = false;
-        if (!(code & TREE_EMPTY)) {
-          const bool enter = valid && slab_enter(lo, hi, L, ix, iy, iz, best);
-          if constexpr (STATS) n_box += valid ? 1ull : 0ull;
-          if (__builtin_amdgcn_ballot_w64(enter) != 0) {
-            if (n < leaf0) {
-              descend = true;
-            } else {
-              refresh();
-              scan_members(n - leaf0, enter);
-            }
+      bool lane_in = false;  // this lane entered the current node's box
+      auto test = [&](const float4& lo, const float4& hi) {
+        lane_in = false;
+        if (__float_as_int(lo.w) & TREE_EMPTY) return false;
+        lane_in = valid && slab_enter(lo, hi, L, ix, iy, iz, best);
+        if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+        return __builtin_amdgcn_ballot_w64(lane_in) != 0;
+      };
+      int32_t n = root, depth = 0;
+      uint32_t firsts = 0, pending = 0;
+      float4 nlo = nodes[2 * root];
+      bool go = test(nlo, nodes[2 * root + 1]);
+      while (go) {  // some lane entered node n
+        if (n >= leaf0) {
+          refresh();
+          scan_members(n - leaf0, lane_in);
+        } else {
+          const int32_t code = __float_as_int(nlo.w);
+          const int32_t near = ((neg >> (code & TREE_AXIS)) & 1) ^ ((code & TREE_SWAP) ? 1 : 0);
+          const float4 alo = nodes[4 * n], ahi = nodes[4 * n + 1], blo = nodes[4 * n + 2], bhi = nodes[4 * n + 3];
+          const bool in_a = test(alo, ahi);
+          const bool lane_a = lane_in;
+          const bool in_b = test(blo, bhi);
+          const bool lane_b = lane_in;
+          const bool in_near = near ? in_b : in_a, in_far = near ? in_a : in_b;
+          if (in_near || in_far) {
+            ++depth;
+            const uint32_t bit = 1u << depth;
+            const bool take_b = in_near ? near != 0 : near == 0;
+            n = 2 * n + (take_b ? 1 : 0);
+            nlo = take_b ? blo : alo;
+            lane_in = take_b ? lane_b : lane_a;
+            if (in_near) firsts |= bit;
+            else firsts &= ~bit;
+            if (in_near && in_far) pending |= bit;
+            else pending &= ~bit;
+            continue;
           }
         }
-        if (descend) {
-          n = first_child(n, code);
-          continue;
+        // climb to the nearest pending sibling on the path and re-test it
+        go = false;
+        while (depth > 0) {
+          const uint32_t bit = 1u << depth;
+          if (firsts & pending & bit) {
+            firsts &= ~bit;
+            pending &= ~bit;
+            n ^= 1;
+            nlo = nodes[2 * n];
+            if (test(nlo, nodes[2 * n + 1])) {
+              go = true;
+              break;
+            }
+            continue;  // now a far child: the next iteration climbs past it
+          }
+          n >>= 1;
+          --depth;
         }
-        // next node: climb while n is its parent's second child
-        bool done = false;
-        for (;;) {
-          if (n == root) { done = true; break; }
-          const int32_t p = n >> 1;
-          if (n == first_child(p, __float_as_int(nodes[2 * p].w))) { n ^= 1; break; }
-          n = p;
-        }
-        if (done) break;
       }
     }
   } else {

@@ -40,7 +40,11 @@ struct BVScene {
   int32_t n_bv, gsize, n_groups;
   const float4* nodes;
   int32_t levels;
+  int32_t walk;  // tree only: BV_WALK_*
 };
+// How a wave walks the tree: together (one path, the union of its rays'; scalar node loads), or
+// each lane its own path; AUTO: together when all its rays point into one octant.
+enum BVWalk { BV_WALK_WAVE = 0, BV_WALK_AUTO = 1, BV_WALK_LANES = 2 };
 enum BVMode { BV_FLAT = 0, BV_GROUPS = 1, BV_TREE = 2 };
 // the kernels' view of a prepared BV hit set
 inline BVScene bv_scene(const splat_hitset_t* hs) {
@@ -56,6 +60,7 @@ inline BVScene bv_scene(const splat_hitset_t* hs) {
   b.n_groups = g ? hs->n_groups : 0;
   b.nodes = tree ? static_cast<const float4*>(hs->d_groups) : nullptr;
   b.levels = tree ? hs->tree_levels - 1 : 0;
+  b.walk = tree && hs->tune_waves >= BV_WALK_WAVE && hs->tune_waves <= BV_WALK_LANES ? hs->tune_waves : BV_WALK_WAVE;
   return b;
 }
 
