@@ -75,6 +75,18 @@ def build_workload(device="cuda"):
     return sc, cam, (bvs, mem), hitter, params, lm, frame
 
 
+def hitter_traffic():
+    """HBM bytes per hitter launch on this workload, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same counters read for a device
+    copy of known size); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_hit_bv_c2_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return float(t["hbm_bytes_per_launch"]), t.get("source")
+
+
 def live_counts(cam, hitter, tris, params) -> list[int]:
     """Live (ACTIVE) rays before each bounce of the frame, A_0 = R, ..., A_depth, from the same
     hitter / evolve / partition operators stepped one bounce at a time (identical rays and
@@ -90,6 +102,57 @@ def live_counts(cam, hitter, tris, params) -> list[int]:
                     optics=params.optics)
         out.append(api.partition_(rays, tr.ray_swap, tr))
     return out
+
+
+def host_info() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def small_frame(budget_s: float = 3.0):
+    """The 320-triangle workload (icosphere subdivision 2 over a plane, 256x256 rays, one seeded
+    wavelength each, brute-force hitter, 2 bounces, 512x512 RGB map): GPU plain-path rate and the
+    oracle's CPU rate on whole frames."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+    from splat_amd import abi, api, scene
+
+    sc = scene.scene_c1()
+    cam = scene.camera_c1(seed=1)
+    tris = api.to_device(sc.ftris)
+    p = abi.run_params(2, sc.first_diffuse, math.nan, seed=2024)
+    lm = abi.lightmap_params(512, 512)
+    fr = api.Frame(cam, api.HIPBruteForceHitter(tris), tris, p, lm)
+    R = fr.R
+    for _ in range(10):
+        fr.trace()
+    torch.cuda.synchronize()
+    K = 200
+    t = time.perf_counter()
+    for _ in range(K):
+        fr.trace()
+    torch.cuda.synchronize()
+    gpu = R * K / (time.perf_counter() - t) / 1e6
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    n, reps = 0, 0
+    t = time.perf_counter()
+    while reps == 0 or (time.perf_counter() - t < budget_s and reps < 50):
+        n += O.trace_sample(cam, 1, 0, sc.ftris, p, lm, None, threads)[1]
+        reps += 1
+    cpu = n / (time.perf_counter() - t) / 1e6
+    return {"workload": "320-triangle icosphere + plane, 256x256 rays, brute force, 2 bounces, 512x512x3 map",
+            "gpu_mrays_per_s": round(gpu, 2), "cpu_mrays_per_s": round(cpu, 3), "cpu_threads": threads,
+            "cpu_frames": reps, "gpu_ms_per_frame": round(R / gpu / 1e3, 4)}
 
 
 def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
@@ -117,7 +180,7 @@ def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
     dt = time.perf_counter() - t
     what = f"every {stride}th ray of the {cam.width}x{cam.height} frame" if stride > 1 else \
         f"the whole {cam.width}x{cam.height} frame x {reps}"
-    return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port", "host": host_info(),
             "sample": f"{what} ({n} rays, {dt:.1f} s): oracle FP32 C++ frame (flat BV culling over "
                       f"2^{depth} object-median leaves, {params.depth} bounces, splat), OpenMP {threads} threads"}
 
@@ -226,6 +289,8 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a 256 MiB device-to-device copy, repeated: calibrates HBM byte counters")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only the 64-frame batch through the engine, --steps times (for a kernel profile)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -250,6 +315,20 @@ def main() -> int:
         return 0
     from splat_amd import api
 
+    if args.profile_batched:
+        from splat_amd import serve
+
+        scenes, reqs = serve.batched_workload(64)
+        arena = int(sum(r.rays for r in reqs) * 440 * 1.05) + (1 << 28)
+        eng = api.Engine(arena, max_slots=64)
+        walls = []
+        for _ in range(max(args.steps, 1)):
+            for r in reqs:
+                r.done = float("nan")
+            walls.append(serve.run_requests(eng, scenes, reqs, realtime=False, max_pending_steps=8).wall_s)
+        eng.close()
+        print(json.dumps({"batches": len(walls), "batched_s": walls}))
+        return 0
     sc, cam, bvmem, hitter, params, lm, frame = build_workload()
     R = frame.R
     for _ in range(args.warmup):
@@ -314,6 +393,7 @@ def main() -> int:
     box_tests, entered, alg_tests, exec_tests = (int(x) for x in stats.cpu().tolist())
     flop_per_launch = (FLOP_PER_BOX_TEST * box_tests + FLOP_PER_TRI_TEST * alg_tests) / n_launch
     achieved = flop_per_launch / avg_launch_s / 1e12
+    traffic, traffic_src = hitter_traffic()
 
     del frame
     torch.cuda.empty_cache()
@@ -323,6 +403,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     serve_1mp = None if args.no_served else served_1mp()
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, params, lm)
+    small = None if args.no_cpu_baseline else small_frame()
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -345,6 +426,7 @@ def main() -> int:
             "bounces": params.depth,
             "path": "plain",
         },
+        "ray_segments_per_s_M": round(sum(live[:D]) * K / wall / 1e6, 3),
         "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "roofline": {
@@ -354,7 +436,8 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "flop_per_launch": flop_per_launch,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
@@ -376,6 +459,7 @@ def main() -> int:
         "served": serve_out,
         "served_1mp": serve_1mp,
         "cpu_baseline": cpu,
+        "small_frame": small,
     }
     print(json.dumps(out))
     return 0
