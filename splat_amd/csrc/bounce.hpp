@@ -62,15 +62,76 @@ __device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in
   return __syncthreads_count(act);
 }
 
-// The bounce pass's LDS: one instance per workgroup, whichever bounce_tile variant runs.
+// The bounce pass's LDS: one instance per workgroup, whichever bounce_tile variant runs. After the
+// tile's records are written out, `stage` holds the privatised light-map bins.
 struct BounceLDS {
   uint2 stage[TILE * WORDS];  // 34,816 bytes: the tile's records
   int16_t order[TILE];
   int32_t wave_a[TILE / 64], wave_r[TILE / 64];
+  int32_t box[TILE / 64][4];  // per wave: min x, max x, min y, max y of its splat texels
 };
+constexpr int SPLAT_BINS = TILE * WORDS * 2;  // floats of `stage`: 8,704
+
+// max/min over the wave (all 64 lanes take part)
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
 __device__ __forceinline__ BounceLDS& bounce_lds() {
   __shared__ BounceLDS lds;
   return lds;
+}
+
+// The tile's pending single-sample splats (has, texel (sx, sy), values sv per plane). When they
+// crowd a small texel window (at least two splats per texel of it on average: caustics), they are
+// summed in LDS bins over that window (the `stage` area, free once the records are written out)
+// and flushed with one global atomic per touched bin; otherwise -- spread-out splats, where a bin
+// would absorb nothing -- they go straight to the map. Called by the whole workgroup after the
+// records' copy-out.
+__device__ __forceinline__ void splat_tile(BounceLDS& lds, bool has, int32_t sx, int32_t sy, const float (&sv)[3],
+                                           const BounceParams& P, const splat_lightmap_params_t& lm) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int np = splat_planes(lm) < 3 ? splat_planes(lm) : 3;
+  const int64_t plane = (int64_t)lm.width * lm.height;
+  const int32_t x0 = wave_min_i32(has ? sx : INT32_MAX), x1 = wave_max_i32(has ? sx : INT32_MIN);
+  const int32_t y0 = wave_min_i32(has ? sy : INT32_MAX), y1 = wave_max_i32(has ? sy : INT32_MIN);
+  if (lane == 0) {
+    lds.box[wave][0] = x0; lds.box[wave][1] = x1; lds.box[wave][2] = y0; lds.box[wave][3] = y1;
+  }
+  const int n_splat = __syncthreads_count(has);  // also: every record has left `stage`
+  int32_t bx0 = INT32_MAX, bx1 = INT32_MIN, by0 = INT32_MAX, by1 = INT32_MIN;
+  for (int w = 0; w < TILE / 64; ++w) {
+    bx0 = min(bx0, lds.box[w][0]); bx1 = max(bx1, lds.box[w][1]);
+    by0 = min(by0, lds.box[w][2]); by1 = max(by1, lds.box[w][3]);
+  }
+  if (bx0 > bx1) return;  // no splat in the tile (uniform)
+  const int64_t ww = (int64_t)bx1 - bx0 + 1, wh = (int64_t)by1 - by0 + 1;
+  if (ww * wh * np > SPLAT_BINS || 2 * ww * wh > n_splat) {  // uniform: straight to the map
+    if (has)
+      for (int c = 0; c < np; ++c)
+        unsafeAtomicAdd(P.tex + sx + (int64_t)lm.width * sy + plane * splat_plane(lm, 0, c), sv[c]);
+    return;
+  }
+  float* bins = reinterpret_cast<float*>(lds.stage);
+  const int nb = (int)(ww * wh * np);
+  for (int e = threadIdx.x; e < nb; e += TILE) bins[e] = 0.0f;
+  __syncthreads();
+  if (has)
+    for (int c = 0; c < np; ++c)
+      atomicAdd(bins + (c * wh + (sy - by0)) * ww + (sx - bx0), sv[c]);
+  __syncthreads();
+  for (int e = threadIdx.x; e < nb; e += TILE) {
+    const float v = bins[e];
+    if (v != 0.0f) {
+      const int c = e / (int)(ww * wh), rem = e - c * (int)(ww * wh);
+      const int32_t y = by0 + rem / (int)ww, x = bx0 + rem % (int)ww;
+      unsafeAtomicAdd(P.tex + x + (int64_t)lm.width * y + plane * splat_plane(lm, 0, c), v);
+    }
+  }
 }
 
 // The bounce pass of one tile (see the top of this file). off_a / off_r: where the tile's ACTIVE
@@ -101,6 +162,13 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
   }
   __syncthreads();
 
+  // One splat sample per ray (one wavelength, no sub-pixel offsets): the tile's splats are summed
+  // in LDS bins over the texel window they cover, then flushed with one global atomic per
+  // touched bin; otherwise every sample goes straight to the map.
+  const bool one_sample = P.tex && lm.n_delta == 1 && lm.n_lambda <= 1;
+  bool has = false;  // this ray has a single-sample splat pending
+  int32_t sx = 0, sy = 0;
+  float sv[3] = {0.f, 0.f, 0.f};
   int32_t i = 1;
   uint8_t placed = 0;  // the status that decides where the ray goes
   if (valid) {
@@ -111,7 +179,18 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
       const FTriF F = load_ftri(P.tris + (i - 1));
       const float u = bounce_random(P.force_rand, nullptr, 0, P.seed, P.bounce, ray.dest);
       RayF o = evolve_one(ray, i, F, u, P.first_diffuse, P.optics);
-      if (P.tex && o.status == SPLAT_RAY_DIFFUSE) final_evolve_splat(o, i, F, lm, P.tex);
+      if (P.tex && o.status == SPLAT_RAY_DIFFUSE) {
+        if (one_sample) {
+          final_evolve(o, i, F);
+          const float lam = lm.n_lambda > 0 ? lm.spectrum[0] : o.lambda;
+          float I;
+          has = splat_sample(o, F, lm, lam, 0, 0, &sx, &sy, &I);
+          if (has)
+            for (int c = 0; c < splat_planes(lm) && c < 3; ++c) sv[c] = splat_value(lm, 0, c, lam, I);
+        } else {
+          final_evolve_splat(o, i, F, lm, P.tex);
+        }
+      }
       store_rayf(rec, o);
     } else {
       placed = ray.status;
@@ -149,6 +228,7 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
     if (slot < na) dst_a[e] = v;
     else dst_r[e - na * WORDS] = v;
   }
+  if (one_sample) splat_tile(lds, has, sx, sy, sv, P, lm);
   __syncthreads();  // the LDS stage is reused by the caller's next tile
 }
 

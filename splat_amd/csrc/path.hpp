@@ -168,11 +168,8 @@ __device__ __forceinline__ float curve_eval(const splat_lightmap_params_t& lm, i
   return FMA(slope, lam - x1, y1);
 }
 
-// final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, in place, then its splats
-// (src/ray_imagers.jl:78-113) in the reference's thread order (k, dy, dx)
-__device__ __forceinline__ void final_evolve_splat(RayF& r, int32_t i, const FTriF& F, const splat_lightmap_params_t& lm,
-                                                   float* __restrict__ tex) {
-  // final_evolution
+// final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, in place
+__device__ __forceinline__ void final_evolve(RayF& r, int32_t i, const FTriF& F) {
   D3<float> tD;
   bool hit = get_hit_ad(r, F, i, &tD);
   if (hit) {
@@ -183,41 +180,69 @@ __device__ __forceinline__ void final_evolve_splat(RayF& r, int32_t i, const FTr
     r.dp[0] = r.dp[1] = r.dp[2] = r.pos;
   }
   r.ignore_tri = hit ? i : 1;
-  // splat (k, dy, dx) in the reference's thread order
-  const int nl = lm.n_lambda > 0 ? lm.n_lambda : 1;
+}
+
+// One splat sample (src/ray_imagers.jl:78-113) of a final-evolved DIFFUSE ray: wavelength k
+// (lam), sub-pixel offsets (ix, iy). Returns false when the texture coordinate is not finite
+// (no splat); else the texel (x, y) (0-based) and the cosine weight I.
+__device__ __forceinline__ bool splat_sample(const RayF& r, const FTriF& F, const splat_lightmap_params_t& lm, float lam,
+                                             int ix, int iy, int32_t* tx, int32_t* ty, float* I) {
   const float W = (float)lm.width, H = (float)lm.height;
+  float xs = r.x + lm.delta[ix], ys = r.y + lm.delta[iy];
+  float dl = lam - r.lambda, dx = xs - r.x, dy = ys - r.y;
+  V3<float> p{FMA(r.dp[2].x, dy, FMA(r.dp[1].x, dx, FMA(r.dp[0].x, dl, r.pos.x))),
+              FMA(r.dp[2].y, dy, FMA(r.dp[1].y, dx, FMA(r.dp[0].y, dl, r.pos.y))),
+              FMA(r.dp[2].z, dy, FMA(r.dp[1].z, dx, FMA(r.dp[0].z, dl, r.pos.z)))};
+  V3<float> d{FMA(r.dd[2].x, dy, FMA(r.dd[1].x, dx, FMA(r.dd[0].x, dl, r.dir.x))),
+              FMA(r.dd[2].y, dy, FMA(r.dd[1].y, dx, FMA(r.dd[0].y, dl, r.dir.y))),
+              FMA(r.dd[2].z, dy, FMA(r.dd[1].z, dx, FMA(r.dd[0].z, dl, r.dir.z)))};
+  float bu, bv;
+  reverse_uv(F, p, &bu, &bv);
+  float w = (1.0f - bu) - bv;
+  float U = FMA(F.v[9].x, w, FMA(F.v[8].x, bv, F.v[7].x * bu));
+  float V = FMA(F.v[9].y, w, FMA(F.v[8].y, bv, F.v[7].y * bu));
+  if (!__builtin_isfinite(U) || !__builtin_isfinite(V)) return false;
+  float fi = ceilf(U * W), fj = ceilf(V * H);
+  fi = fi < 1.0f ? 1.0f : (fi > W ? W : fi);
+  fj = fj < 1.0f ? 1.0f : (fj > H ? H : fj);
+  *tx = (int32_t)fi - 1;
+  *ty = (int32_t)fj - 1;
+  V3<float> nrm = optical_normal(F, p);
+  *I = fabsf(dot3(nrm, normalize3(d)));
+  return true;
+}
+
+// Light-map planes a sample of wavelength k writes, and the value for each: plane k with I
+// (identity weights), or each channel c with I times the table or curve weight of lam.
+__device__ __forceinline__ int splat_planes(const splat_lightmap_params_t& lm) {
+  return lm.weight_mode == SPLAT_WEIGHTS_IDENTITY ? 1 : lm.channels;
+}
+__device__ __forceinline__ int32_t splat_plane(const splat_lightmap_params_t& lm, int k, int c) {
+  return lm.weight_mode == SPLAT_WEIGHTS_IDENTITY ? k : c;
+}
+__device__ __forceinline__ float splat_value(const splat_lightmap_params_t& lm, int k, int c, float lam, float I) {
+  if (lm.weight_mode == SPLAT_WEIGHTS_IDENTITY) return I;
+  if (lm.weight_mode == SPLAT_WEIGHTS_TABLE) return I * lm.weights[k * 3 + c];
+  return I * curve_eval(lm, c, lam);
+}
+
+// final_evolution of one DIFFUSE ray, in place, then all its splats (k, dy, dx) straight into
+// the map with global atomics, in the reference's thread order
+__device__ __forceinline__ void final_evolve_splat(RayF& r, int32_t i, const FTriF& F, const splat_lightmap_params_t& lm,
+                                                   float* __restrict__ tex) {
+  final_evolve(r, i, F);
+  const int nl = lm.n_lambda > 0 ? lm.n_lambda : 1;
   const int64_t plane = (int64_t)lm.width * lm.height;
   for (int k = 0; k < nl; ++k) {
     const float lam = lm.n_lambda > 0 ? lm.spectrum[k] : r.lambda;
     for (int iy = 0; iy < lm.n_delta; ++iy) {
       for (int ix = 0; ix < lm.n_delta; ++ix) {
-        float xs = r.x + lm.delta[ix], ys = r.y + lm.delta[iy];
-        float dl = lam - r.lambda, dx = xs - r.x, dy = ys - r.y;
-        V3<float> p{FMA(r.dp[2].x, dy, FMA(r.dp[1].x, dx, FMA(r.dp[0].x, dl, r.pos.x))),
-                    FMA(r.dp[2].y, dy, FMA(r.dp[1].y, dx, FMA(r.dp[0].y, dl, r.pos.y))),
-                    FMA(r.dp[2].z, dy, FMA(r.dp[1].z, dx, FMA(r.dp[0].z, dl, r.pos.z)))};
-        V3<float> d{FMA(r.dd[2].x, dy, FMA(r.dd[1].x, dx, FMA(r.dd[0].x, dl, r.dir.x))),
-                    FMA(r.dd[2].y, dy, FMA(r.dd[1].y, dx, FMA(r.dd[0].y, dl, r.dir.y))),
-                    FMA(r.dd[2].z, dy, FMA(r.dd[1].z, dx, FMA(r.dd[0].z, dl, r.dir.z)))};
-        float bu, bv;
-        reverse_uv(F, p, &bu, &bv);
-        float w = (1.0f - bu) - bv;
-        float U = FMA(F.v[9].x, w, FMA(F.v[8].x, bv, F.v[7].x * bu));
-        float V = FMA(F.v[9].y, w, FMA(F.v[8].y, bv, F.v[7].y * bu));
-        if (!__builtin_isfinite(U) || !__builtin_isfinite(V)) continue;
-        float fi = ceilf(U * W), fj = ceilf(V * H);
-        fi = fi < 1.0f ? 1.0f : (fi > W ? W : fi);
-        fj = fj < 1.0f ? 1.0f : (fj > H ? H : fj);
-        const int64_t base = ((int64_t)fi - 1) + (int64_t)lm.width * ((int64_t)fj - 1);
-        V3<float> nrm = optical_normal(F, p);
-        float I = fabsf(dot3(nrm, normalize3(d)));
-        if (lm.weight_mode == SPLAT_WEIGHTS_IDENTITY) {
-          unsafeAtomicAdd(tex + base + plane * k, I);
-        } else if (lm.weight_mode == SPLAT_WEIGHTS_TABLE) {
-          for (int c = 0; c < lm.channels; ++c) unsafeAtomicAdd(tex + base + plane * c, I * lm.weights[k * 3 + c]);
-        } else {
-          for (int c = 0; c < lm.channels; ++c) unsafeAtomicAdd(tex + base + plane * c, I * curve_eval(lm, c, lam));
-        }
+        int32_t tx, ty;
+        float I;
+        if (!splat_sample(r, F, lm, lam, ix, iy, &tx, &ty, &I)) continue;
+        const int64_t base = (int64_t)tx + (int64_t)lm.width * ty;
+        for (int c = 0; c < splat_planes(lm); ++c)
+          unsafeAtomicAdd(tex + base + plane * splat_plane(lm, k, c), splat_value(lm, k, c, lam, I));
       }
     }
   }
