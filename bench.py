@@ -3,8 +3,8 @@
 
 Workload: two displaced icospheres (subdivision 4 each, 10,240 glass triangles, radial noise
 sigma 0.02) over a diffuse plane; 1024 x 1024 rays from a pinhole at the point light, one seeded
-wavelength per ray, Cauchy glass (A 1.5, B 0.004 um^2), bounding-volume hitter (tree over 2^10
-object-median leaves), 4
+wavelength per ray, Cauchy glass (A 1.5, B 0.004 um^2), bounding-volume hitter (ground triangles
+tested directly, tree over 2^10 object-median leaves of the rest), 4
 bounces with dual-number differentials, splat into a 1024 x 1024 RGB float32 light map.
 A step is one whole frame: clear map -> pinhole rays -> bounce loop (hit, evolve, stable
 compaction) -> realigning hit -> final evolution + splat, all device-resident, one frame per
@@ -66,9 +66,9 @@ def build_workload(device="cuda"):
 
     sc = scene.scene_c2(seed=2)
     cam = scene.camera_c2(seed=2, jitter=0.025)
-    bvs, mem = bv.object_partition(sc.ftris, bv.tree_depth(sc.n_tris))
+    bvs, mem, n_top = bv.tree_partition(sc.ftris)
     tris_d = api.to_device(sc.ftris, device)
-    hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=0)
+    hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=0, n_top=n_top)
     params = abi.run_params(4, sc.first_diffuse, math.nan, seed=2024)
     lm = abi.lightmap_params(1024, 1024)
     frame = api.Frame(cam, hitter, tris_d, params, lm, device)
@@ -419,7 +419,7 @@ def main() -> int:
         "data": "synthetic: seeded procedural scene and pinhole rays (no dataset)",
         "config": {
             "workload": "1 MP light-map frame: 2 displaced icospheres (subdiv 4, 10,240 glass tris) + diffuse plane, "
-                        "BV hitter (binary tree over 2^10 object-median leaves), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
+                        "BV hitter (the 2 ground triangles tested directly, binary tree over 2^10 object-median leaves of the rest), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
                         "light map; plain path (one frame per launch sequence)",
             "rays_per_frame": R,
             "triangles": sc.n_tris,

@@ -129,6 +129,8 @@ typedef struct splat_hitset {
                          /* BV tree: 0 = waves walk together, 1 = per wave, together when */
                          /* its rays share a direction octant, else per lane; 2 = per lane */
   int32_t tree_levels;   /* 1 + levels of the tree over the BVs; 0 = no tree (BV only)    */
+  int32_t top_bvs;       /* leading BVs whose members every ray tests directly (BV only) */
+  int32_t top_recs;      /* their records: d_recs[0, top_recs)                            */
   /* Optional device uint64[4] counters, accumulated by every hitter launch when non-NULL:
    * [0] ray-box slab tests, [1] (ray, box) pairs entered, [2] ray-triangle tests of the
    * algorithm (brute force: every live ray x every triangle; BV: the members of the boxes each
@@ -230,12 +232,14 @@ int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t n_tris_le
  *                    walked depth-first, nearer child first, by each wavefront (default);
  * group_size > 0  -- one coarse level of boxes over `group_size` consecutive BVs;
  * group_size < 0  -- nothing: every BV box is tested (the reference's one-level structure).
- * Any memberships that cover the triangles give the brute-force result. */
+ * The first n_top BVs are excluded from that structure: every ray tests their members directly,
+ * first (meant for a few large triangles -- a ground plane -- that would otherwise inflate every
+ * box above them). Any memberships that cover the triangles give the brute-force result. */
 size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members);
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t n_tris_len, int32_t n_bv,
                                 const int64_t* bv_tri_count, const int64_t* bv_tris,
-                                int64_t bv_stride, int32_t group_size, void* ws, size_t ws_bytes,
-                                splat_hitset_t* out, void* stream);
+                                int64_t bv_stride, int32_t group_size, int32_t n_top, void* ws,
+                                size_t ws_bytes, splat_hitset_t* out, void* stream);
 
 /* next_hit!(tracer, hitter, rays, n_tris) (src/hitters.jl:24-30, :34-82, :168-226, :615-648):
  * hit_idx[r] = the lexicographic minimum over triangles j of (t_rj, j), ties to the lowest

@@ -35,8 +35,9 @@ mutable struct HitSet
     total_members::Int64
     d_recs::Ptr{Cvoid}; d_boxes::Ptr{Cvoid}; d_ranges::Ptr{Cvoid}; d_groups::Ptr{Cvoid}
     group_size::Int32; n_groups::Int32; tune_waves::Int32; tree_levels::Int32
+    top_bvs::Int32; top_recs::Int32
     d_stats::Ptr{UInt64}
-    HitSet() = new(0, 0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, C_NULL)
+    HitSet() = new(0, 0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, 0, 0, 0, 0, 0, 0, C_NULL)
 end
 
 struct RunParams
@@ -66,15 +67,16 @@ struct HIPBVHitter <: AbstractHitter
     ws::ROCArray{UInt8}
 end
 # bvs/memberships as returned by bv_partition (src/bv.jl:92-121); packed like pack_bv_tris
-function HIPBVHitter(tris::ROCArray{FTri}, bvs, memberships; group_size = 0)
+function HIPBVHitter(tris::ROCArray{FTri}, bvs, memberships; group_size = 0, n_top = 0)
     counts, bv_tris = pack_bv_tris(Array, Array(tris), bvs, memberships)  # host copies
     n_bv = length(bvs)
     total = sum(counts)
     ws = ROCArray{UInt8}(undef, ccall((:splat_hitset_bv_bytes, LIB), Csize_t, (Int32, Int64), n_bv, total))
     hs = HitSet()
     check(ccall((:splat_hitset_prepare_bv, LIB), Int32,
-                (Ptr{Cvoid}, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Int64, Int32, Ptr{Cvoid}, Csize_t, Ref{HitSet}, Ptr{Cvoid}),
-                pointer(tris), length(tris), n_bv, counts, bv_tris, size(bv_tris, 2), group_size,
+                (Ptr{Cvoid}, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Int64, Int32, Int32, Ptr{Cvoid}, Csize_t, Ref{HitSet},
+                 Ptr{Cvoid}),
+                pointer(tris), length(tris), n_bv, counts, bv_tris, size(bv_tris, 2), group_size, n_top,
                 pointer(ws), length(ws), hs, stream_ptr()), "prepare_bv")
     HIPBVHitter(hs, ws)
 end

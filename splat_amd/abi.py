@@ -63,7 +63,8 @@ class HitSet(C.Structure):
         ("kind", C.c_int32), ("n_tris", C.c_int32), ("n_bv", C.c_int32), ("max_members", C.c_int32),
         ("total_members", C.c_int64), ("d_recs", C.c_void_p), ("d_boxes", C.c_void_p),
         ("d_ranges", C.c_void_p), ("d_groups", C.c_void_p), ("group_size", C.c_int32), ("n_groups", C.c_int32),
-        ("tune_waves", C.c_int32), ("tree_levels", C.c_int32), ("d_stats", C.c_void_p),
+        ("tune_waves", C.c_int32), ("tree_levels", C.c_int32), ("top_bvs", C.c_int32), ("top_recs", C.c_int32),
+        ("d_stats", C.c_void_p),
     ]
 
 
@@ -105,7 +106,7 @@ class FrameRequest(C.Structure):
 
 
 PARAM_STRUCTS = [Optics, HitSet, LightMapParams, Pinhole, RunParams, FrameRequest]
-assert C.sizeof(Pinhole) == 96 and C.sizeof(RunParams) == 56 and C.sizeof(HitSet) == 80
+assert C.sizeof(Pinhole) == 96 and C.sizeof(RunParams) == 56 and C.sizeof(HitSet) == 88
 
 
 def optics(model: int = IOR_CAUCHY, a: float = 1.5, b: float = 0.004, compat: int = COMPAT_REFERENCE) -> Optics:

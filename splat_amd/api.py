@@ -82,10 +82,11 @@ class HIPBruteForceHitter:
 class HIPBVHitter:
     """One-level bounding-volume hitter: DPBVHitter(A, ray_length, tris, bvs, memberships)
     (src/alg_types.jl:81-90, src/hitters.jl:615-648). Memberships are packed as pack_bv_tris
-    does (src/bv.jl:126-136); the result equals the brute-force hitter's."""
+    does (src/bv.jl:126-136); the result equals the brute-force hitter's. The first n_top
+    memberships are tested by every ray directly (bv.tree_partition puts large triangles there)."""
 
     def __init__(self, tris, bvs=None, memberships=None, depth: int = 6, host_tris: np.ndarray | None = None,
-                 group_size: int = 0):
+                 group_size: int = 0, n_top: int = 0):
         L.require_gpu()
         _check_rows(tris, abi.FTRI.itemsize, "tris")
         if memberships is None:
@@ -101,7 +102,7 @@ class HIPBVHitter:
         self.hs = abi.HitSet()
         L.check(lib.splat_hitset_prepare_bv(L.ptr(tris), _nrows(tris), counts.shape[0],
                                             counts.ctypes.data_as(C.c_void_p),
-                                            self.mat.ctypes.data_as(C.c_void_p), stride, group_size, L.ptr(self.ws),
+                                            self.mat.ctypes.data_as(C.c_void_p), stride, group_size, n_top, L.ptr(self.ws),
                                             self.ws.numel(), L.ref(self.hs), L.stream_handle()), "prepare_bv")
 
 

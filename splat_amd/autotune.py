@@ -6,8 +6,9 @@ the fastest is kept, and choices are cached in memory and, optionally, in a JSON
 
 Tuned knobs:
   brute force -- waves per workgroup W that split the triangle list (1, 2, 4, 8, 16);
-  BV          -- leaf count of the object-median build (2^d) and the top structure over the
-                 leaves (binary tree, none, or a coarse level of fixed-size groups).
+  BV          -- leaf count of the object-median build (2^d; large triangles apart, tested
+                 directly) and the top structure over the leaves (binary tree, none, or a coarse
+                 level of fixed-size groups).
 All candidates return identical hit indices (the tuner asserts it on the tuning rays).
 """
 from __future__ import annotations
@@ -92,9 +93,9 @@ def tune_bv(tris_dev, host_tris: np.ndarray, rays, cache_path: str | None = None
     if key not in _cache:
         best, ref = None, None
         for d in depths:
-            bvs, mem = bv.object_partition(host_tris, d)
+            bvs, mem, n_top = bv.tree_partition(host_tris, d)
             for g in group_sizes:
-                h = api.HIPBVHitter(tris_dev, bvs, mem, group_size=g)
+                h = api.HIPBVHitter(tris_dev, bvs, mem, group_size=g, n_top=n_top)
                 ms, hits = _time_hits(h, rays)
                 if ref is None:
                     ref = hits
@@ -105,6 +106,6 @@ def tune_bv(tris_dev, host_tris: np.ndarray, rays, cache_path: str | None = None
         _cache[key] = best
         _save(cache_path)
     choice = dict(_cache[key])
-    bvs, mem = bv.object_partition(host_tris, choice["depth"])
-    choice["hitter"] = api.HIPBVHitter(tris_dev, bvs, mem, group_size=choice["group_size"])
+    bvs, mem, n_top = bv.tree_partition(host_tris, choice["depth"])
+    choice["hitter"] = api.HIPBVHitter(tris_dev, bvs, mem, group_size=choice["group_size"], n_top=n_top)
     return choice

@@ -85,6 +85,35 @@ def object_partition(ftris: np.ndarray, depth: int = 8):
     return bvs, members
 
 
+def tree_partition(ftris: np.ndarray, depth: int | None = None, big_factor: float = 16.0, max_big: int = 32):
+    """The build for the tree hitter: triangles whose bounding-box diagonal exceeds big_factor x
+    the median (a ground plane; at most max_big of the largest) go into one leading membership
+    that every ray tests directly; the rest are split object-median into 2**depth leaves
+    (depth = tree_depth of their count when None). A large triangle inside the tree would
+    inflate its leaf's box and every box above it, so every ray would enter them. Returns
+    (bvs, memberships, n_top) with n_top = 1 when such triangles exist, else 0."""
+    v = ftris["v"][1:, 1:4, :].astype(np.float64)
+    diag = np.linalg.norm(v.max(axis=1) - v.min(axis=1), axis=1)
+    med = float(np.median(diag)) if diag.size else 0.0
+    big = np.nonzero(diag > big_factor * med)[0] if med > 0 else np.zeros(0, dtype=np.int64)
+    if big.size > max_big:
+        big = big[np.argsort(diag[big])[::-1][:max_big]]
+    big = np.sort(big) + 2  # 1-based triangle indices (row 0 is the sentinel)
+    rest = np.setdiff1d(np.arange(2, ftris.shape[0] + 1, dtype=np.int64), big)
+    if depth is None:
+        depth = tree_depth(rest.size)
+    sub = ftris[np.concatenate([[0], rest - 1])]  # sentinel + the rest, re-indexed 2..
+    bvs_r, mem_r = object_partition(sub, depth)
+    mem_r = [rest[m - 2] for m in mem_r]
+    if big.size == 0:
+        return bvs_r, mem_r, 0
+    vb = ftris["v"][big - 1][:, 1:4, :].reshape(-1, 3)
+    top = np.zeros(1, dtype=abi.BVBOX)
+    top[0]["min"] = vb.min(axis=0)
+    top[0]["max"] = vb.max(axis=0)
+    return np.concatenate([top, bvs_r]), [big] + mem_r, 1
+
+
 def pack_bv_tris(memberships: list[np.ndarray], n_tris: int, max_overcount_factor: float = 4.0):
     """pack_bv_tris (src/bv.jl:126-136): dense n_bv x stride matrix of 1-based indices, Julia
     column-major (element (k, m) at k + m*n_bv), plus the per-BV counts. The stride grows to the

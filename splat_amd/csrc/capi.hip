@@ -317,9 +317,10 @@ size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members) {
 }
 
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_bv, const int64_t* bv_tri_count,
-                                const int64_t* bv_tris, int64_t bv_stride, int32_t group_size, void* ws, size_t ws_bytes,
-                                splat_hitset_t* out, void* stream) {
+                                const int64_t* bv_tris, int64_t bv_stride, int32_t group_size, int32_t n_top, void* ws,
+                                size_t ws_bytes, splat_hitset_t* out, void* stream) {
   if (!tris || N < 1 || n_bv < 1 || !bv_tri_count || !bv_tris || bv_stride < 0 || !ws || !out) return SPLAT_ERR_INVALID_ARG;
+  if (n_top < 0 || n_top >= n_bv) return SPLAT_ERR_INVALID_ARG;  // at least one BV under the structure
   if (tree_levels_for(n_bv) > MAX_TREE_LEVELS) return SPLAT_ERR_UNSUPPORTED;
   // flatten the Julia column-major (n_bv x stride) membership matrix; drop the sentinel index 1
   std::vector<int32_t> members;
@@ -341,10 +342,12 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   }
   const int64_t total = (int64_t)members.size();
   if (ws_bytes < splat_hitset_bv_bytes(n_bv, total)) return SPLAT_ERR_WORKSPACE;
+  // the structure covers BVs [n_top, n_bv); the first n_top BVs' records lead d_recs
+  const int32_t m = n_bv - n_top;
   const bool tree = group_size == 0;
-  const int32_t levels = tree_levels_for(n_bv);
-  if (group_size < 0 || n_bv < 4) group_size = 0;
-  const int32_t n_groups = group_size > 0 ? (n_bv + group_size - 1) / group_size : 0;
+  const int32_t levels = tree_levels_for(m);
+  if (group_size < 0 || m < 4) group_size = 0;
+  const int32_t n_groups = group_size > 0 ? (m + group_size - 1) / group_size : 0;
   Carver c(ws);
   TriRec* recs = c.take<TriRec>((size_t)total);
   float4* boxes = c.take<float4>((size_t)n_bv * 2);
@@ -358,8 +361,8 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
     return SPLAT_ERR_HIP;
   launch_prep_members(tris, d_members, total, recs, s);
   launch_bv_boxes(tris, d_members, d_ranges, n_bv, boxes, s);
-  if (tree) launch_tree_build(boxes, d_ranges, n_bv, levels, groups, s);
-  else if (n_groups > 0) launch_group_boxes(boxes, n_bv, group_size, groups, s);
+  if (tree) launch_tree_build(boxes + 2 * n_top, d_ranges + n_top, m, levels, groups, s);
+  else if (n_groups > 0) launch_group_boxes(boxes + 2 * n_top, m, group_size, groups, s);
   if (int32_t e = launch_status()) return e;
   // the host vectors must outlive the copies
   if (hipStreamSynchronize(s) != hipSuccess) return SPLAT_ERR_HIP;
@@ -376,6 +379,8 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   out->group_size = tree ? 0 : group_size;
   out->n_groups = tree ? 0 : n_groups;
   out->tree_levels = tree ? levels + 1 : 0;
+  out->top_bvs = n_top;
+  out->top_recs = n_top > 0 ? ranges[(size_t)n_top].x : 0;
   return SPLAT_OK;
 }
 

@@ -242,6 +242,24 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       }
     }
   };
+  // the records every ray tests before the structure (large triangles)
+  if (S.n_top > 0) {
+    const int32_t nt = S.n_top;
+    if constexpr (STATS) {
+      n_alg += valid ? (unsigned long long)nt : 0ull;
+      n_exec += (unsigned long long)nt;
+    }
+    for (int32_t m = 0; m < nt; ++m) {
+      const float4 q0 = recs[m].q0, q1 = recs[m].q1, q2 = recs[m].q2, q3 = recs[m].q3;
+      float t;
+      bool h = hit_rec(q0, q1, q2, q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
+      int32_t idx = __float_as_int(q1.w);
+      if (h && (t < best || (t == best && idx < bi))) {
+        best = t;
+        bi = idx;
+      }
+    }
+  }
   auto scan_leaf = [&](int32_t b) {
     const float4 blo = boxes[2 * b], bhi = boxes[2 * b + 1];
     const bool enter = valid && slab_enter(blo, bhi, L, ix, iy, iz, best);

@@ -40,7 +40,8 @@ struct BVScene {
   int32_t n_bv, gsize, n_groups;
   const float4* nodes;
   int32_t levels;
-  int32_t walk;  // tree only: BV_WALK_*
+  int32_t walk;   // tree only: BV_WALK_*
+  int32_t n_top;  // records recs[0, n_top) that every ray tests before the structure
 };
 // How a wave walks the tree: together (one path, the union of its rays'; scalar node loads), or
 // each lane its own path; AUTO: together when all its rays point into one octant.
@@ -49,10 +50,12 @@ enum BVMode { BV_FLAT = 0, BV_GROUPS = 1, BV_TREE = 2 };
 // the kernels' view of a prepared BV hit set
 inline BVScene bv_scene(const splat_hitset_t* hs) {
   BVScene b;
-  b.boxes = static_cast<const float4*>(hs->d_boxes);
-  b.ranges = static_cast<const int2*>(hs->d_ranges);
+  const int32_t top = hs->top_bvs > 0 && hs->top_bvs < hs->n_bv ? hs->top_bvs : 0;
+  b.boxes = static_cast<const float4*>(hs->d_boxes) + 2 * top;
+  b.ranges = static_cast<const int2*>(hs->d_ranges) + top;
   b.recs = static_cast<const TriRec*>(hs->d_recs);
-  b.n_bv = hs->n_bv;
+  b.n_bv = hs->n_bv - top;
+  b.n_top = top > 0 ? hs->top_recs : 0;
   const bool tree = hs->d_groups && hs->tree_levels > 0;
   const bool g = !tree && hs->d_groups && hs->group_size > 0 && hs->n_groups > 0;
   b.groups = g ? static_cast<const float4*>(hs->d_groups) : nullptr;

@@ -27,11 +27,9 @@ class ResidentScene:
 
     @staticmethod
     def build(sc: scene.Scene, depth: int | None = None, device: str = "cuda") -> "ResidentScene":
-        if depth is None:
-            depth = bv.tree_depth(sc.n_tris)
         tris = api.to_device(sc.ftris, device)
-        bvs, mem = bv.object_partition(sc.ftris, depth)
-        return ResidentScene(sc, tris, api.HIPBVHitter(tris, bvs, mem, group_size=0), sc.name)
+        bvs, mem, n_top = bv.tree_partition(sc.ftris, depth)
+        return ResidentScene(sc, tris, api.HIPBVHitter(tris, bvs, mem, group_size=0, n_top=n_top), sc.name)
 
 
 @dataclass

@@ -23,10 +23,13 @@ def _sync():
 @pytest.fixture(scope="module")
 def scenes():
     out = []
-    for sc, depth in ((scene.scene_c1(), 3), (H.small_c2(2), 4), (H.small_c2(3), 5)):
+    for k, (sc, depth) in enumerate(((scene.scene_c1(), 3), (H.small_c2(2), 4), (H.small_c2(3), 5))):
         tris = api.to_device(sc.ftris)
-        bvs, mem = bv.object_partition(sc.ftris, depth)
-        out.append((sc, tris, api.HIPBVHitter(tris, bvs, mem, group_size=0)))
+        if k == 1:  # ground triangles tested directly, tree over the rest
+            bvs, mem, n_top = bv.tree_partition(sc.ftris, depth)
+        else:
+            (bvs, mem), n_top = bv.object_partition(sc.ftris, depth), 0
+        out.append((sc, tris, api.HIPBVHitter(tris, bvs, mem, group_size=0, n_top=n_top)))
     return out
 
 

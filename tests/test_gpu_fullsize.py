@@ -23,8 +23,8 @@ def _frame(sc, cam, depth, lm, hitter_kind="bv", bv_depth=8, seed=2024):
 
     tris = api.to_device(sc.ftris)
     if hitter_kind == "bv":
-        bvs, mem = bv.object_partition(sc.ftris, bv_depth)
-        hitter = api.HIPBVHitter(tris, bvs, mem, group_size=0)
+        bvs, mem, n_top = bv.tree_partition(sc.ftris, bv_depth)
+        hitter = api.HIPBVHitter(tris, bvs, mem, group_size=0, n_top=n_top)
     else:
         hitter = api.HIPBruteForceHitter(tris)
     p = abi.run_params(depth, sc.first_diffuse, math.nan, seed=seed)

@@ -146,6 +146,28 @@ def test_next_hit_bv_equals_bf(build, depth, group_size):
     assert (got == want).all()
 
 
+@pytest.mark.parametrize("depth,group_size,walk", [(3, 0, 0), (5, 0, 1), (5, 0, 2), (4, 4, 0), (4, -1, 0), (1, 0, 0)])
+def test_next_hit_bv_top_list_equals_bf(depth, group_size, walk):
+    """Large triangles tested directly before the structure (tree_partition), every structure
+    and tree walk: the brute-force result."""
+    sc = H.small_c2(4)
+    cam = scene.camera_c2(seed=4, height=64, width=64)
+    rays = O.raygen(cam)
+    pts = H.sample_surface_points(sc.ftris, 2000, seed=15)
+    rays = np.concatenate([rays, H.rays_towards(pts, seed=16), H.random_rays(1000, seed=17, n_tris=sc.n_tris)])
+    want, _ = O.next_hit(rays, sc.ntris())
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem, n_top = bv.tree_partition(sc.ftris, depth)
+    assert n_top == 1 and mem[0].size == 2  # the ground quad
+    hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=group_size, n_top=n_top)
+    hitter.hs.tune_waves = walk
+    rays_d = api.to_device(rays)
+    tr = api.HIPTracer(rays_d)
+    api.next_hit_(tr, hitter, rays_d)
+    _sync()
+    assert (tr.hit_idx.cpu().numpy() == want).all()
+
+
 # ---- evolve ---------------------------------------------------------------------------------
 
 def _evolve_gpu(rays_np, hit, ftris, **kw):
