@@ -182,12 +182,12 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
   P.seed = p->seed;
   P.optics = p->optics;
   P.tex = lm ? tex : nullptr;
-  launch_set_i64(&w.counters[0], R, s);
   splat_adray_t* cur = w.s1;
   splat_adray_t* nxt = w.s2;
   uint64_t* kc = w.keys;   // this bounce's keys
   uint64_t* kn = w.keys2;  // the next bounce's, set to the miss key by the bounce pass
-  hit_keys_init(R, &w.counters[0], hs, kc, s);
+  launch_frame_start(lm ? tex : nullptr, lm ? (int64_t)lm->width * lm->height * lm->channels : 0, &w.counters[0], R,
+                     bv ? kc : nullptr, s);
   for (int k = 1; k <= p->depth; ++k) {
     const int64_t* live = &w.counters[k - 1];
     const splat_pinhole_t* gen = k == 1 ? cam : nullptr;
@@ -214,9 +214,8 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
     mark(2 * p->depth);
     hit_keys_launch(cur, R, live, hs, kc, s);  // kc holds the miss keys of the live rays
     mark(2 * p->depth + 1);
-    launch_key_final(kc, R, live, hit_final, s);
   }
-  launch_copy_rays(cur, final_rays, R, live, s);
+  launch_frame_end(cur, final_rays, R, live, bv ? kc : nullptr, hit_final, s);
   return launch_status();
 }
 
@@ -493,8 +492,8 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
     return SPLAT_ERR_INVALID_ARG;
   RunWs w = carve_run(ws, R);
   hipStream_t s = S(stream);
-  launch_fill_f32(tex, (int64_t)lm->width * lm->height * lm->channels, 0.0f, s);
-  // the first bounce generates its rays itself; only a frame without bounces needs them stored
+  // (the light map is cleared by run_core) the first bounce generates its rays itself; only a
+  // frame without bounces needs them stored
   if (p->depth == 0) launch_raygen(*cam, w.s1, R, s);
   // the light map is splatted inside the loop, as rays become DIFFUSE
   if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s, lm, tex, p->depth > 0 ? cam : nullptr)) return e;

@@ -81,6 +81,11 @@ struct BounceParams;  // bounce.hpp
 // cam != nullptr: the live rays are that pinhole's rays, generated on the fly (first bounce)
 void launch_bounce_count(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
                          const BounceParams& P, const splat_pinhole_t* cam, int32_t* tile_counts, hipStream_t s);
+// tex (nullable) cleared, *count0 = R, keys (nullable) set to the miss key
+void launch_frame_start(float* tex, int64_t tex_n, int64_t* count0, int64_t R, uint64_t* keys, hipStream_t s);
+// live rays src -> dst; hit_idx from keys when keys != nullptr
+void launch_frame_end(const splat_adray_t* src, splat_adray_t* dst, int64_t R, const int64_t* d_count,
+                      const uint64_t* keys, int32_t* hit_idx, hipStream_t s);
 // lm == nullptr: no splat (P.tex ignored); keys_next (nullable): miss keys at the next positions
 void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
                    uint64_t* keys_next, const BounceParams& P, const splat_lightmap_params_t* lm,

@@ -8,6 +8,8 @@ namespace spl {
 
 // ---- fused bounce loop passes (bounce.hpp) ----------------------------------------------------
 // from_cam: the live rays are the pinhole's rays 0..R-1, not yet generated (the first bounce)
+// Count pass: ACTIVE rays per live tile after this bounce (a separate one-workgroup scan follows:
+// a last-workgroup scan here would need one same-address atomic per tile, which serialises)
 __global__ __launch_bounds__(TILE) void k_bounce_count(const splat_adray_t* __restrict__ rays, int64_t R,
                                                        const int64_t* __restrict__ d_count,
                                                        const uint64_t* __restrict__ keys, BounceParams P,
@@ -20,6 +22,33 @@ __global__ __launch_bounds__(TILE) void k_bounce_count(const splat_adray_t* __re
   const int c = from_cam ? bounce_count<true>(rays + base, keys + base, nvalid, P, true, cam, base)
                          : bounce_count<false>(rays + base, keys + base, nvalid, P, true, cam, base);
   if (threadIdx.x == 0) tile_counts[blockIdx.x] = c;
+}
+
+// Start of a frame's loop: light map (when given) cleared, live count R, the first bounce's miss
+// keys (when given).
+__global__ void k_frame_start(float* __restrict__ tex, int64_t tex_n, int64_t* __restrict__ count0, int64_t R,
+                              uint64_t* __restrict__ keys) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tex)
+    for (int64_t i = g; i < tex_n; i += stride) tex[i] = 0.0f;
+  if (keys)
+    for (int64_t i = g; i < R; i += stride) keys[i] = KEY_MISS;
+  if (g == 0) *count0 = R;
+}
+
+// End of the loop: the live rays copied to their final slots (8-byte words, coalesced) and, when
+// keys are given, their realigning hit indices taken from them.
+__global__ void k_frame_end(const uint2* __restrict__ src, uint2* __restrict__ dst, int64_t R,
+                            const int64_t* __restrict__ d_count, const uint64_t* __restrict__ keys,
+                            int32_t* __restrict__ hit_idx) {
+  const int64_t A = live_count_t(R, d_count);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = A * (int64_t)(sizeof(splat_adray_t) / 8);
+  for (int64_t i = g; i < n; i += stride) dst[i] = src[i];
+  if (keys)
+    for (int64_t i = g; i < A; i += stride) hit_idx[i] = key_index(keys[i]);
 }
 
 __global__ __launch_bounds__(TILE) void k_bounce(const splat_adray_t* __restrict__ rays, int64_t R,
@@ -115,6 +144,20 @@ void launch_bounce_count(const splat_adray_t* rays, int64_t R, const int64_t* d_
   else std::memset(&c, 0, sizeof(c));
   hipLaunchKernelGGL(k_bounce_count, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, keys,
                      P, c, cam ? 1 : 0, tile_counts);
+}
+static unsigned frame_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (unsigned)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
+}
+void launch_frame_start(float* tex, int64_t tex_n, int64_t* count0, int64_t R, uint64_t* keys, hipStream_t s) {
+  const int64_t n = (tex ? tex_n : 0) > (keys ? R : 0) ? (tex ? tex_n : 0) : (keys ? R : 0);
+  hipLaunchKernelGGL(k_frame_start, dim3(frame_grid(n)), dim3(256), 0, s, tex, tex_n, count0, R, keys);
+}
+void launch_frame_end(const splat_adray_t* src, splat_adray_t* dst, int64_t R, const int64_t* d_count,
+                      const uint64_t* keys, int32_t* hit_idx, hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(k_frame_end, dim3(frame_grid(R * 17)), dim3(256), 0, s, (const uint2*)src, (uint2*)dst, R,
+                     d_count, keys, hit_idx);
 }
 void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
                    uint64_t* keys_next, const BounceParams& P, const splat_lightmap_params_t* lm,
