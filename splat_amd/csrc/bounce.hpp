@@ -178,6 +178,8 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
       placed = predict_status(i, P.first_diffuse, P.tris, ray.pos, ray.dir, ray.ignore_tri);
       const FTriF F = load_ftri(P.tris + (i - 1));
       const float u = bounce_random(P.force_rand, nullptr, 0, P.seed, P.bounce, ray.dest);
+      // (final_evolution repeats evolve's hit test rather than keeping its dual distance live
+      // across evolve: measured faster, the registers are the scarce resource here)
       RayF o = evolve_one(ray, i, F, u, P.first_diffuse, P.optics);
       if (P.tex && o.status == SPLAT_RAY_DIFFUSE) {
         if (one_sample) {

@@ -385,13 +385,16 @@ namespace {
 
 int32_t launch_step_kernels(splat_engine* e, int64_t hit_blocks, hipStream_t s) {
   const EngineDev& E = e->dev;
+  // the tile-item stages get about one workgroup per tile (hit_blocks >= the tile count), so
+  // workgroups at different phases of their tile overlap on a CU as in the single-frame passes
+  const unsigned work = (unsigned)(hit_blocks < e->grid ? e->grid : (hit_blocks > 65536 ? 65536 : hit_blocks));
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
-  hipLaunchKernelGGL(k_b_raygen, dim3(e->grid), dim3(256), 0, s, E);
+  hipLaunchKernelGGL(k_b_raygen, dim3(work), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
-  hipLaunchKernelGGL(k_b_count, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_count, dim3(work), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
-  hipLaunchKernelGGL(k_b_bounce, dim3(e->grid), dim3(TILE), 0, s, E);
-  hipLaunchKernelGGL(k_b_final, dim3(e->grid), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_bounce, dim3(work), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_final, dim3(work), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_finish, dim3(1), dim3(1024), 0, s, E);
   return hipGetLastError() == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
 }

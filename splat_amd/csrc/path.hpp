@@ -168,10 +168,9 @@ __device__ __forceinline__ float curve_eval(const splat_lightmap_params_t& lm, i
   return FMA(slope, lam - x1, y1);
 }
 
-// final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, in place
-__device__ __forceinline__ void final_evolve(RayF& r, int32_t i, const FTriF& F) {
-  D3<float> tD;
-  bool hit = get_hit_ad(r, F, i, &tD);
+// final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, in place, given its hit test on
+// triangle i (hit, dual distance tD)
+__device__ __forceinline__ void final_evolve_hit(RayF& r, int32_t i, bool hit, const D3<float>& tD) {
   if (hit) {
     V3<D3<float>> P = next_p_dual(pos_dual(r), dir_dual(r), tD);
     split(P, &r.pos, r.dp);
@@ -180,6 +179,11 @@ __device__ __forceinline__ void final_evolve(RayF& r, int32_t i, const FTriF& F)
     r.dp[0] = r.dp[1] = r.dp[2] = r.pos;
   }
   r.ignore_tri = hit ? i : 1;
+}
+__device__ __forceinline__ void final_evolve(RayF& r, int32_t i, const FTriF& F) {
+  D3<float> tD;
+  const bool hit = get_hit_ad(r, F, i, &tD);
+  final_evolve_hit(r, i, hit, tD);
 }
 
 // One splat sample (src/ray_imagers.jl:78-113) of a final-evolved DIFFUSE ray: wavelength k
