@@ -309,6 +309,27 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
                           const splat_lightmap_params_t* lm, float* tex, void* ws,
                           size_t ws_bytes, void* stream);
 
+/* ---- backward pass: camera rays -> image (the rest of trace!, src/tracer.jl:346-385) ------ */
+/* run_evolution!(...; reorder = true)'s reorder (src/tracer.jl:270-278): ray r and its hit index
+ * move to slot r.dest - 1, i.e. back to pixel order (dest must be a permutation of 1..R; a dest
+ * outside 1..R is dropped). Workspace: splat_reorder_ws_bytes(R). */
+size_t splat_reorder_ws_bytes(int64_t R);
+int32_t splat_reorder_by_dest(splat_adray_t* rays, int64_t R, int32_t* hit_idx, void* ws, size_t ws_bytes,
+                              void* stream);
+/* continuum_shade!(::ExperimentalImager2; tracer, RGB, tris, rays, spectrum, retina_factor, tex,
+ * skydome) (src/ray_imagers.jl:440-503, shade_tex4_kernel :381-436) for rays in pixel order on a
+ * grid of grid_h rows (ray q is pixel (q % grid_h, q / grid_h)): DIFFUSE rays are final-evolved in
+ * place; then per ray and sub-pixel offset (a, b) of lm->delta: an INFINITY ray reads the skydome
+ * (3 float planes sky_w x sky_h, u fastest) at u = acos(d_z)/pi * sky_w, v = (atan2(d_y, d_x)/pi
+ * + 1)/2 * sky_h; a DIFFUSE ray sums, over the lm->n_lambda planes of the light map `tex` at its
+ * texture coordinate, lm->weights[k*3 + c] x the plane's value; channels clamped to [0, 1].
+ * Textures are read bilinearly at 1-based coordinates, clamped at the edges (CUDA.jl's linear
+ * texture reads, with exact FP32 weights). rgb: (grid_h * n_delta) x (R / grid_h * n_delta)
+ * pixels, 3 interleaved floats each, row index fastest (the reference's RGB layout). */
+int32_t splat_shade(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris,
+                    int32_t n_tris_len, const splat_lightmap_params_t* lm, const float* tex, const float* sky,
+                    int32_t sky_w, int32_t sky_h, int64_t grid_h, float* rgb, void* stream);
+
 /* ---- multi-frame engine: batching and continuous serving --------------------------------
  * Frames (each with its own resident scene, camera, depth and light map) occupy slots of an
  * engine. One step advances every admitted frame by one stage -- raygen + bounce 1, bounce k,

@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -645,6 +646,101 @@ void splat_one(const RayT<T>& r, const FTriT<T>& F, const splat_lightmap_params_
 }
 
 // ------------------------------------------------------------------------------------------
+// backward shading: continuum_shade! / shade_tex4_kernel (src/ray_imagers.jl:381-503)
+// ------------------------------------------------------------------------------------------
+// Bilinear sample of one float plane (width W along u, height H along v, u fastest) at 1-based
+// coordinates (u, v) with clamp-to-edge: the reference reads its light map and skydome through
+// CUDA.jl textures with linear interpolation, where an integer coordinate i hits texel i exactly
+// and a fraction a blends texels floor(u) and floor(u) + 1. The weights are exact FP32 here
+// (texture hardware rounds them to 8 fractional bits: a documented, bounded difference).
+template <class T> inline T tex_bilinear(const float* plane, int64_t W, int64_t H, T u, T v) {
+  T fu = std::floor(u), fv = std::floor(v);
+  T a = u - fu, b = v - fv;
+  fu = fu < T(0) ? T(0) : (fu > T(W) ? T(W) : fu);
+  fv = fv < T(0) ? T(0) : (fv > T(H) ? T(H) : fv);
+  int64_t i0 = (int64_t)fu, j0 = (int64_t)fv;
+  int64_t i1 = i0 + 1 > W ? W : i0 + 1, j1 = j0 + 1 > H ? H : j0 + 1;
+  i0 = i0 < 1 ? 1 : i0;
+  j0 = j0 < 1 ? 1 : j0;
+  i1 = i1 < 1 ? 1 : i1;
+  j1 = j1 < 1 ? 1 : j1;
+  T t00 = T(plane[(i0 - 1) + W * (j0 - 1)]), t10 = T(plane[(i1 - 1) + W * (j0 - 1)]);
+  T t01 = T(plane[(i0 - 1) + W * (j1 - 1)]), t11 = T(plane[(i1 - 1) + W * (j1 - 1)]);
+  T r0 = FMA(a, t10 - t00, t00), r1 = FMA(a, t11 - t01, t01);
+  return FMA(b, r1 - r0, r0);
+}
+
+// One (ray, dx, dy) sample of shade_tex4_kernel (src/ray_imagers.jl:381-436) for a final-evolved
+// ray r on triangle F: an INFINITY ray reads the skydome at its direction's polar angles (u from
+// the polar angle, v from the azimuth), a DIFFUSE ray sums retina-weighted light-map values at
+// its texture coordinate over the spectrum; channels clamped to [0, 1].
+template <class T>
+void shade_sample(const RayT<T>& r, const FTriT<T>& F, const splat_lightmap_params_t& lm, const float* tex,
+                  const float* sky, int64_t SW, int64_t SH, int a, int b, T rgb[3]) {
+  rgb[0] = rgb[1] = rgb[2] = T(0);
+  const T pi = T((float)3.14159265358979323846);
+  if (r.status == SPLAT_RAY_INFINITY && sky) {
+    T xs = r.x + T(lm.delta[a]), ys = r.y + T(lm.delta[b]);
+    T dx = xs - r.x, dy = ys - r.y;  // expand(adr, adr.lambda, x, y): no lambda step
+    V3<T> d{FMA(r.dd[2].x, dy, FMA(r.dd[1].x, dx, r.dir.x)), FMA(r.dd[2].y, dy, FMA(r.dd[1].y, dx, r.dir.y)),
+            FMA(r.dd[2].z, dy, FMA(r.dd[1].z, dx, r.dir.z))};
+    T th = std::acos(d.z), ph = std::atan2(d.y, d.x);
+    T v = ((ph / pi) + T(1)) * T(0.5), u = th / pi;
+    u = u * T((float)SW);
+    v = v * T((float)SH);
+    if (std::isfinite(u) && std::isfinite(v))
+      for (int c = 0; c < 3; ++c) rgb[c] = tex_bilinear<T>(sky + SW * SH * c, SW, SH, u, v);
+  } else if (r.status == SPLAT_RAY_DIFFUSE) {
+    const T W = T((float)lm.width), H = T((float)lm.height);
+    for (int k = 0; k < lm.n_lambda; ++k) {
+      T lam = T(lm.spectrum[k]);
+      T xs = r.x + T(lm.delta[a]), ys = r.y + T(lm.delta[b]);
+      T dl = lam - r.lambda, dx = xs - r.x, dy = ys - r.y;
+      V3<T> p{FMA(r.dp[2].x, dy, FMA(r.dp[1].x, dx, FMA(r.dp[0].x, dl, r.pos.x))),
+              FMA(r.dp[2].y, dy, FMA(r.dp[1].y, dx, FMA(r.dp[0].y, dl, r.pos.y))),
+              FMA(r.dp[2].z, dy, FMA(r.dp[1].z, dx, FMA(r.dp[0].z, dl, r.pos.z)))};
+      T bu, bv;
+      reverse_uv(F, p, &bu, &bv);
+      T w = (T(1) - bu) - bv;
+      T U = FMA(F.v[9].x, w, FMA(F.v[8].x, bv, F.v[7].x * bu));
+      T V = FMA(F.v[9].y, w, FMA(F.v[8].y, bv, F.v[7].y * bu));
+      if (std::isnan(U) || std::isnan(V)) continue;
+      T X = tex_bilinear<T>(tex + (int64_t)lm.width * lm.height * k, lm.width, lm.height, U * W, V * H);
+      for (int c = 0; c < 3; ++c) rgb[c] = FMA(T(lm.weights[k * 3 + c]), X, rgb[c]);
+    }
+  }
+  for (int c = 0; c < 3; ++c) rgb[c] = rgb[c] > T(1) ? T(1) : (rgb[c] < T(0) ? T(0) : rgb[c]);
+}
+
+// continuum_shade! (src/ray_imagers.jl:440-503) over rays in pixel order (ray q is pixel
+// (q % h + 1, q / h + 1) of an h-row grid): final_evolution in place, then each ray's n_delta^2
+// samples into the (h * n_delta)-row RGB image, interleaved, row index fastest.
+template <class T, class O>
+void shade_frame(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const std::vector<FTriT<T>>& F, int32_t N,
+                 const splat_lightmap_params_t& lm, const float* tex, const float* sky, int64_t SW, int64_t SH,
+                 int64_t h, O* rgb, bool store_rays) {
+  const int nd = lm.n_delta;
+  const int64_t img_h = h * nd;
+#pragma omp parallel for schedule(static, 1024)
+  for (int64_t q = 0; q < R; ++q) {
+    int32_t i = hit_idx[q];
+    if (i < 1 || i > N) i = 1;
+    RayT<T> fe = final_evolution_one<T>(load_ray<T>(rays[q]), i, F[i - 1]);
+    if (store_rays) {
+      if constexpr (std::is_same<T, float>::value) store_ray(rays[q], fe);
+    }
+    const int64_t x = q % h, y = q / h;
+    for (int b = 0; b < nd; ++b)
+      for (int a = 0; a < nd; ++a) {
+        T c3[3];
+        shade_sample<T>(fe, F[i - 1], lm, tex, sky, SW, SH, a, b, c3);
+        const int64_t X = a + nd * x, Y = b + nd * y;
+        for (int c = 0; c < 3; ++c) rgb[(X + img_h * Y) * 3 + c] = (O)c3[c];
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // seeded random numbers (the reference's CUDA.rand is unseeded, src/tracer.jl:219,
 // src/ray_generators.jl:57): splitmix64's finaliser over (seed, stream, counter)
 // ------------------------------------------------------------------------------------------
@@ -1031,6 +1127,36 @@ int64_t oracle_trace_sample_f32(const splat_pinhole_t* cam, int64_t R_full, int6
     splat_one<float, float>(fe, F[i - 1], *lm, tex);
   }
   return R;
+}
+
+// backward shading of rays in pixel order (continuum_shade!, src/ray_imagers.jl:440-503)
+void oracle_shade_f32(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris, int32_t N,
+                      const splat_lightmap_params_t* lm, const float* tex, const float* sky, int32_t sky_w,
+                      int32_t sky_h, int64_t h, float* rgb, int32_t threads) {
+  set_threads(threads);
+  std::vector<FTriT<float>> F = load_ftris<float>(tris, N);
+  shade_frame<float, float>(rays, R, hit_idx, F, N, *lm, tex, sky, sky_w, sky_h, h, rgb, true);
+}
+void oracle_shade_f64(const splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris, int32_t N,
+                      const splat_lightmap_params_t* lm, const float* tex, const float* sky, int32_t sky_w,
+                      int32_t sky_h, int64_t h, double* rgb, int32_t threads) {
+  set_threads(threads);
+  std::vector<FTriT<double>> F = load_ftris<double>(tris, N);
+  std::vector<splat_adray_t> tmp(rays, rays + R);
+  shade_frame<double, double>(tmp.data(), R, hit_idx, F, N, *lm, tex, sky, sky_w, sky_h, h, rgb, false);
+}
+double oracle_bilinear_f64(const float* plane, int64_t W, int64_t H, double u, double v) {
+  return tex_bilinear<double>(plane, W, H, u, v);
+}
+
+// run_evolution!'s reorder (src/tracer.jl:270-278): ray and hit index to slot dest - 1
+void oracle_reorder(splat_adray_t* rays, int64_t R, int32_t* hit_idx) {
+  std::vector<splat_adray_t> t(rays, rays + R);
+  std::vector<int32_t> h(hit_idx, hit_idx + R);
+  for (int64_t r = 0; r < R; ++r) {
+    int64_t d = t[r].dest;
+    if (d >= 1 && d <= R) { rays[d - 1] = t[r]; hit_idx[d - 1] = h[r]; }
+  }
 }
 
 }  // extern "C"

@@ -35,6 +35,10 @@ def lib():
         i32, i64, u64, f32, f64 = C.c_int32, C.c_int64, C.c_uint64, C.c_float, C.c_double
         sig = {
             "oracle_mix64": (u64, [u64]),
+            "oracle_shade_f32": (None, [P, i64, P, P, i32, P, P, P, i32, i32, i64, P, i32]),
+            "oracle_shade_f64": (None, [P, i64, P, P, i32, P, P, P, i32, i32, i64, P, i32]),
+            "oracle_bilinear_f64": (f64, [P, i64, i64, f64, f64]),
+            "oracle_reorder": (None, [P, i64, P]),
             "oracle_u01": (f32, [u64, u64, u64]),
             "oracle_key32": (C.c_uint32, [f32]),
             "oracle_key64": (u64, [f32, C.c_uint32]),
@@ -211,3 +215,41 @@ def trace_sample(cam, stride, offset, ftris, params, lm, bv=None, threads=0):
                                           _ref(lm), _p(tex), counts.shape[0], _p(counts),
                                           mat.ctypes.data_as(C.c_void_p), st, threads)
     return tex, int(n)
+
+
+def shade(rays, hit_idx, ftris, lm, tex, sky, h, precision="f32", threads=0):
+    """continuum_shade! (src/ray_imagers.jl:440-503) of rays in pixel order on an h-row grid:
+    returns (rgb image as (h*nd, w*nd) pixel rows x 3, row index fastest; the final-evolved rays
+    (f32 only))."""
+    hit_idx = np.ascontiguousarray(hit_idx, dtype=np.int32)
+    R = rays.shape[0]
+    nd = lm.n_delta
+    tex = np.ascontiguousarray(tex, dtype=np.float32)
+    if sky is None:
+        sky_p, sw, sh = None, 0, 0
+    else:
+        sky = np.ascontiguousarray(sky, dtype=np.float32)
+        sky_p, sw, sh = _p(sky), sky.shape[2], sky.shape[1]
+    if precision == "f32":
+        rays = np.array(rays, copy=True)
+        rgb = np.zeros((R * nd * nd, 3), dtype=np.float32)
+        lib().oracle_shade_f32(_p(rays), R, _p(hit_idx), _p(ftris), ftris.shape[0], _ref(lm), _p(tex), sky_p, sw, sh,
+                               h, _p(rgb), threads)
+        return rgb, rays
+    rgb = np.zeros((R * nd * nd, 3), dtype=np.float64)
+    lib().oracle_shade_f64(_p(np.ascontiguousarray(rays)), R, _p(hit_idx), _p(ftris), ftris.shape[0], _ref(lm),
+                           _p(tex), sky_p, sw, sh, h, _p(rgb), threads)
+    return rgb, None
+
+
+def bilinear(plane, u, v):
+    plane = np.ascontiguousarray(plane, dtype=np.float32)
+    return lib().oracle_bilinear_f64(_p(plane), plane.shape[1], plane.shape[0], float(u), float(v))
+
+
+def reorder(rays, hit_idx):
+    """run_evolution!'s reorder (src/tracer.jl:270-278): each ray and hit index to slot dest - 1."""
+    rays = np.array(rays, copy=True)
+    hit_idx = np.array(hit_idx, dtype=np.int32, copy=True)
+    lib().oracle_reorder(_p(rays), rays.shape[0], _p(hit_idx))
+    return rays, hit_idx

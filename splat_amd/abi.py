@@ -161,12 +161,25 @@ def retina_curves() -> list[tuple[np.ndarray, np.ndarray]]:
     return out
 
 
+def retina_factor(spectrum) -> np.ndarray:
+    """retina_factor of _spectrum_datastructs (src/rgb_spectrum.jl:135-148): the normalised
+    retina responses (red, green, blue) at each wavelength, float32, shape (n_lambda, 3)."""
+    out = np.zeros((len(spectrum), 3), dtype=np.float32)
+    for c, name in enumerate(("red", "green", "blue")):
+        pts = _RETINA[name]
+        norm = sum(_piecewise64(pts, float(x)) for x in range(400, 701))
+        for k, lam in enumerate(spectrum):
+            out[k, c] = np.float32(_piecewise64(pts, float(lam)) / norm)
+    return out
+
+
 def lightmap_params(width: int, height: int, *, n_delta: int = 1, spectrum=None, mode: str = "rgb") -> LightMapParams:
     """Build splat parameters.
 
     spectrum=None  -> own-lambda mode (each ray splats at its own wavelength).
     mode "rgb"     -> 3 channels weighted by the retina curves at the splat's lambda.
-    mode "spectral"-> one plane per spectrum entry, weight 1 (the reference's tex[i,j,k]).
+    mode "spectral"-> one plane per spectrum entry, weight 1 (the reference's tex[i,j,k]); the
+                      weights table holds the retina factors, which backward shading reads.
     mode "mono"    -> one plane, weight 1 (own-lambda only).
     """
     p = LightMapParams()
@@ -199,6 +212,10 @@ def lightmap_params(width: int, height: int, *, n_delta: int = 1, spectrum=None,
             raise ValueError("spectral mode needs a spectrum grid")
         p.channels = p.n_lambda
         p.weight_mode = WEIGHTS_IDENTITY
+        rf = retina_factor(spectrum)
+        for k in range(spectrum.size):
+            for c in range(3):
+                p.weights[k * 3 + c] = float(rf[k, c])
     elif mode == "mono":
         if spectrum is not None:
             raise ValueError("mono mode is own-lambda only")

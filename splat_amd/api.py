@@ -221,6 +221,61 @@ def rays_from_pinhole(cam: abi.Pinhole, device: str = "cuda"):
     return rays
 
 
+def reorder_(rays, tracer: HIPTracer) -> None:
+    """run_evolution!(...; reorder = true)'s reorder (src/tracer.jl:270-278): rays and
+    tracer.hit_idx back to pixel order (slot dest - 1)."""
+    _check_rows(rays, abi.ADRAY.itemsize, "rays")
+    R = _nrows(rays)
+    ws = _ws(L.lib().splat_reorder_ws_bytes(R), rays.device)
+    L.check(L.lib().splat_reorder_by_dest(L.ptr(rays), R, L.ptr(tracer.hit_idx), L.ptr(ws), ws.numel(),
+                                          L.stream_handle()), "reorder")
+
+
+def continuum_shade_(*, tracer: HIPTracer, tris, rays, tex, lm: abi.LightMapParams, grid_h: int, skydome=None,
+                     rgb=None):
+    """continuum_shade!(::ExperimentalImager2; tracer, RGB, tris, rays, spectrum, retina_factor,
+    tex, skydome) (src/ray_imagers.jl:440-503): rays in pixel order on a grid_h-row grid ->
+    RGB image (grid_h*n_delta x R/grid_h*n_delta pixels, row index fastest) as an (n_pixels, 3)
+    float32 tensor. tex: the spectral light map (lm.n_lambda planes); lm.weights: the retina
+    factors; skydome: (3, sky_h, sky_w) float32 or None."""
+    torch = _torch()
+    _check_rows(rays, abi.ADRAY.itemsize, "rays")
+    _check_rows(tris, abi.FTRI.itemsize, "tris")
+    R = _nrows(rays)
+    nd = lm.n_delta
+    if rgb is None:
+        rgb = torch.empty((R * nd * nd, 3), dtype=torch.float32, device=rays.device)
+    sw = sh = 0
+    if skydome is not None:
+        sh, sw = int(skydome.shape[1]), int(skydome.shape[2])
+    L.check(L.lib().splat_shade(L.ptr(rays), R, L.ptr(tracer.hit_idx), L.ptr(tris), _nrows(tris), L.ref(lm),
+                                L.ptr(tex), L.ptr(skydome), sw, sh, grid_h, L.ptr(rgb), L.stream_handle()), "shade")
+    return rgb
+
+
+def trace_(*, forward_hitter, backward_hitter, cam: abi.Pinhole, light: abi.Pinhole, tris, depth: int,
+           first_diffuse: int, spectrum, tex_width: int, tex_height: int, skydome=None, seed: int = 0,
+           force_rand=None, upscale: int = 1):
+    """trace! (src/tracer.jl:291-385), one iteration: the forward light map -- rays from the
+    light pinhole, run_evolution!, continuum_light_map2! into one plane per spectrum entry --
+    then the backward image -- camera rays (one per (upscale x upscale) pixel block, lambda 550),
+    run_evolution!(reorder = true), continuum_shade! with upscale^2 sub-pixel offsets.
+    Returns (rgb (n_pixels, 3) float32, light map)."""
+    lm = abi.lightmap_params(tex_width, tex_height, spectrum=spectrum, mode="spectral")
+    fr_p = abi.run_params(depth, first_diffuse, math.nan if force_rand is None else float(force_rand), seed=seed)
+    fwd = Frame(light, forward_hitter, tris, fr_p, lm)
+    fwd.trace()
+    rays = rays_from_pinhole(cam)
+    tr = HIPTracer(rays, upscale)
+    run_evolution_(backward_hitter, tr, depth=depth, first_diffuse=first_diffuse, tris=tris, rays=rays,
+                   force_rand=force_rand, seed=seed + 1)
+    reorder_(rays, tr)
+    shade_lm = abi.lightmap_params(tex_width, tex_height, n_delta=upscale, spectrum=spectrum, mode="spectral")
+    rgb = continuum_shade_(tracer=tr, tris=tris, rays=rays, tex=fwd.tex, lm=shade_lm, grid_h=cam.height,
+                           skydome=skydome)
+    return rgb, fwd.tex
+
+
 class Frame:
     """Device-resident state of one light-map frame: rays, tracer arrays, workspace, light map.
     `trace(...)` runs pinhole -> run_evolution! -> continuum_light_map2! with no host sync."""

@@ -500,4 +500,40 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
   return launch_status();
 }
 
+size_t splat_reorder_ws_bytes(int64_t R) {
+  if (R < 0) return 0;
+  return align_up((size_t)R * sizeof(splat_adray_t)) + align_up((size_t)(R > 0 ? R : 1) * sizeof(int32_t));
+}
+
+int32_t splat_reorder_by_dest(splat_adray_t* rays, int64_t R, int32_t* hit_idx, void* ws, size_t ws_bytes,
+                              void* stream) {
+  if (R < 0 || (R > 0 && (!rays || !hit_idx || !ws))) return SPLAT_ERR_INVALID_ARG;
+  if (R == 0) return SPLAT_OK;
+  if (ws_bytes < splat_reorder_ws_bytes(R)) return SPLAT_ERR_WORKSPACE;
+  Carver c(ws);
+  splat_adray_t* tr = c.take<splat_adray_t>((size_t)R);
+  int32_t* th = c.take<int32_t>((size_t)R);
+  hipStream_t s = S(stream);
+  if (hipMemcpyAsync(tr, rays, (size_t)R * sizeof(splat_adray_t), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(th, hit_idx, (size_t)R * sizeof(int32_t), hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return SPLAT_ERR_HIP;
+  launch_reorder(tr, th, R, rays, hit_idx, s);
+  return launch_status();
+}
+
+int32_t splat_shade(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris, int32_t N,
+                    const splat_lightmap_params_t* lm, const float* tex, const float* sky, int32_t sky_w, int32_t sky_h,
+                    int64_t grid_h, float* rgb, void* stream) {
+  if (R < 0 || !tris || N < 1 || !lm || grid_h < 1) return SPLAT_ERR_INVALID_ARG;
+  if (R == 0) return SPLAT_OK;
+  if (!rays || !hit_idx || !rgb || R % grid_h != 0) return SPLAT_ERR_INVALID_ARG;
+  if (lm->width < 1 || lm->height < 1 || lm->n_delta < 1 || lm->n_delta > SPLAT_MAX_DELTA || lm->n_lambda < 0 ||
+      lm->n_lambda > SPLAT_MAX_LAMBDA)
+    return SPLAT_ERR_INVALID_ARG;
+  if (lm->n_lambda > 0 && !tex) return SPLAT_ERR_INVALID_ARG;
+  if (sky && (sky_w < 1 || sky_h < 1)) return SPLAT_ERR_INVALID_ARG;
+  launch_shade(rays, R, hit_idx, tris, N, *lm, tex, sky, sky ? sky_w : 0, sky ? sky_h : 0, grid_h, rgb, S(stream));
+  return launch_status();
+}
+
 }  // extern "C"

@@ -98,6 +98,11 @@ void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const
 void launch_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,
                       const splat_ftri_t* tris, int32_t N, const splat_lightmap_params_t& lm, float* tex,
                       hipStream_t s);
+void launch_shade(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris, int32_t N,
+                  const splat_lightmap_params_t& lm, const float* tex, const float* sky, int32_t SW, int32_t SH,
+                  int64_t h, float* rgb, hipStream_t s);
+void launch_reorder(const splat_adray_t* src, const int32_t* hsrc, int64_t R, splat_adray_t* dst, int32_t* hdst,
+                    hipStream_t s);
 void launch_raygen(const splat_pinhole_t& cam, splat_adray_t* rays, int64_t R, hipStream_t s);
 void launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s);
 void launch_set_i64(int64_t* dst, int64_t v, hipStream_t s);

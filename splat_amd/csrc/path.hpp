@@ -260,6 +260,68 @@ __device__ __forceinline__ void final_evolve_and_splat(RayF r, int32_t i, const 
   store_rayf(ray_out, r);
 }
 
+// ---- backward shading: shade_tex4_kernel (src/ray_imagers.jl:381-436) ------------------------
+// Bilinear sample of one float plane (W along u, H along v, u fastest) at 1-based (u, v), clamp
+// to edge: the reference's CUDA.jl linear-interpolation texture reads, with exact FP32 weights.
+__device__ __forceinline__ float tex_bilinear(const float* __restrict__ plane, int64_t W, int64_t H, float u, float v) {
+  float fu = floorf(u), fv = floorf(v);
+  const float a = u - fu, b = v - fv;
+  fu = fu < 0.f ? 0.f : (fu > (float)W ? (float)W : fu);
+  fv = fv < 0.f ? 0.f : (fv > (float)H ? (float)H : fv);
+  int64_t i0 = (int64_t)fu, j0 = (int64_t)fv;
+  int64_t i1 = i0 + 1 > W ? W : i0 + 1, j1 = j0 + 1 > H ? H : j0 + 1;
+  i0 = i0 < 1 ? 1 : i0;
+  j0 = j0 < 1 ? 1 : j0;
+  i1 = i1 < 1 ? 1 : i1;
+  j1 = j1 < 1 ? 1 : j1;
+  const float t00 = plane[(i0 - 1) + W * (j0 - 1)], t10 = plane[(i1 - 1) + W * (j0 - 1)];
+  const float t01 = plane[(i0 - 1) + W * (j1 - 1)], t11 = plane[(i1 - 1) + W * (j1 - 1)];
+  const float r0 = FMA(a, t10 - t00, t00), r1 = FMA(a, t11 - t01, t01);
+  return FMA(b, r1 - r0, r0);
+}
+
+// One (ray, a, b) sample for a final-evolved ray r on triangle F: an INFINITY ray reads the
+// skydome (3 planes SW x SH) at its direction's polar angles; a DIFFUSE ray sums the retina-
+// weighted (lm.weights[k*3 + c]) light-map values over the spectrum at its texture coordinate;
+// channels clamped to [0, 1].
+__device__ __forceinline__ void shade_sample(const RayF& r, const FTriF& F, const splat_lightmap_params_t& lm,
+                                             const float* __restrict__ tex, const float* __restrict__ sky, int64_t SW,
+                                             int64_t SH, int a, int b, float rgb[3]) {
+  rgb[0] = rgb[1] = rgb[2] = 0.f;
+  const float pi = 3.14159265358979323846f;
+  if (r.status == SPLAT_RAY_INFINITY && sky) {
+    const float xs = r.x + lm.delta[a], ys = r.y + lm.delta[b];
+    const float dx = xs - r.x, dy = ys - r.y;
+    const V3<float> d{FMA(r.dd[2].x, dy, FMA(r.dd[1].x, dx, r.dir.x)), FMA(r.dd[2].y, dy, FMA(r.dd[1].y, dx, r.dir.y)),
+                      FMA(r.dd[2].z, dy, FMA(r.dd[1].z, dx, r.dir.z))};
+    const float th = acosf(d.z), ph = atan2f(d.y, d.x);
+    float v = ((ph / pi) + 1.f) * 0.5f, u = th / pi;
+    u = u * (float)SW;
+    v = v * (float)SH;
+    if (__builtin_isfinite(u) && __builtin_isfinite(v))
+      for (int c = 0; c < 3; ++c) rgb[c] = tex_bilinear(sky + SW * SH * c, SW, SH, u, v);
+  } else if (r.status == SPLAT_RAY_DIFFUSE) {
+    const float W = (float)lm.width, H = (float)lm.height;
+    for (int k = 0; k < lm.n_lambda; ++k) {
+      const float lam = lm.spectrum[k];
+      const float xs = r.x + lm.delta[a], ys = r.y + lm.delta[b];
+      const float dl = lam - r.lambda, dx = xs - r.x, dy = ys - r.y;
+      const V3<float> p{FMA(r.dp[2].x, dy, FMA(r.dp[1].x, dx, FMA(r.dp[0].x, dl, r.pos.x))),
+                        FMA(r.dp[2].y, dy, FMA(r.dp[1].y, dx, FMA(r.dp[0].y, dl, r.pos.y))),
+                        FMA(r.dp[2].z, dy, FMA(r.dp[1].z, dx, FMA(r.dp[0].z, dl, r.pos.z)))};
+      float bu, bv;
+      reverse_uv(F, p, &bu, &bv);
+      const float w = (1.0f - bu) - bv;
+      const float U = FMA(F.v[9].x, w, FMA(F.v[8].x, bv, F.v[7].x * bu));
+      const float V = FMA(F.v[9].y, w, FMA(F.v[8].y, bv, F.v[7].y * bu));
+      if (__builtin_isnan(U) || __builtin_isnan(V)) continue;
+      const float X = tex_bilinear(tex + (int64_t)lm.width * lm.height * k, lm.width, lm.height, U * W, V * H);
+      for (int c = 0; c < 3; ++c) rgb[c] = FMA(lm.weights[k * 3 + c], X, rgb[c]);
+    }
+  }
+  for (int c = 0; c < 3; ++c) rgb[c] = rgb[c] > 1.f ? 1.f : (rgb[c] < 0.f ? 0.f : rgb[c]);
+}
+
 // camera_ray (src/ray_generators.jl:18-50) for ray q of a pinhole frame, evaluated in Float64 and
 // rounded to the Float32 ADRay fields as the reference does
 __device__ __forceinline__ splat_adray_t raygen_ray(const splat_pinhole_t& cam, int64_t q) {

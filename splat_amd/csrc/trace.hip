@@ -123,6 +123,53 @@ __global__ __launch_bounds__(256) void k_raygen(const splat_pinhole_t cam, splat
   raygen_one(cam, q, rays + q);
 }
 
+// ---- backward shading (continuum_shade!, src/ray_imagers.jl:440-503) ----------------------------
+// one thread per ray (rays in pixel order on an h-row grid): final_evolution in place, then its
+// n_delta^2 samples into the (h * n_delta)-row interleaved RGB image
+__global__ __launch_bounds__(256) void k_shade(splat_adray_t* __restrict__ rays, int64_t R,
+                                               const int32_t* __restrict__ hit_idx,
+                                               const splat_ftri_t* __restrict__ tris, int32_t N,
+                                               const splat_lightmap_params_t lm, const float* __restrict__ tex,
+                                               const float* __restrict__ sky, int32_t SW, int32_t SH, int64_t h,
+                                               float* __restrict__ rgb) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= R) return;
+  RayF r = load_rayf(rays + q);
+  int32_t i = hit_idx[q];
+  if (i < 1 || i > N) i = 1;
+  const FTriF F = load_ftri(tris + (i - 1));
+  if (r.status == SPLAT_RAY_DIFFUSE) {
+    final_evolve(r, i, F);
+    store_rayf(rays + q, r);
+  }
+  const int nd = lm.n_delta;
+  const int64_t img_h = h * nd, x = q % h, y = q / h;
+  for (int b = 0; b < nd; ++b)
+    for (int a = 0; a < nd; ++a) {
+      float c3[3];
+      shade_sample(r, F, lm, tex, sky, SW, SH, a, b, c3);
+      float* o = rgb + ((a + nd * x) + img_h * (b + nd * y)) * 3;
+      o[0] = c3[0];
+      o[1] = c3[1];
+      o[2] = c3[2];
+    }
+}
+
+// run_evolution!'s reorder (src/tracer.jl:270-278): record and hit index of ray r to slot dest - 1
+// (a dest outside 1..R is skipped)
+__global__ __launch_bounds__(256) void k_reorder(const splat_adray_t* __restrict__ src, const int32_t* __restrict__ hsrc,
+                                                 int64_t R, splat_adray_t* __restrict__ dst, int32_t* __restrict__ hdst) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int64_t d = src[r].dest;
+  if (d < 1 || d > R) return;
+  const uint2* s = reinterpret_cast<const uint2*>(src + r);
+  uint2* t = reinterpret_cast<uint2*>(dst + (d - 1));
+#pragma unroll
+  for (int w = 0; w < (int)(sizeof(splat_adray_t) / 8); ++w) t[w] = s[w];
+  hdst[d - 1] = hsrc[r];
+}
+
 // ---- small utilities (graph-capturable: plain kernels, no memset nodes) ----------------------
 __global__ void k_fill_f32(float* __restrict__ dst, int64_t n, float v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = v;
@@ -188,6 +235,18 @@ void launch_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count, co
                       int32_t N, const splat_lightmap_params_t& lm, float* tex, hipStream_t s) {
   if (R <= 0) return;
   hipLaunchKernelGGL(k_light_map, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, rays, R, d_count, hit_idx, tris, N, lm, tex);
+}
+void launch_shade(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris, int32_t N,
+                  const splat_lightmap_params_t& lm, const float* tex, const float* sky, int32_t SW, int32_t SH,
+                  int64_t h, float* rgb, hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(k_shade, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, rays, R, hit_idx, tris, N, lm, tex, sky,
+                     SW, SH, h, rgb);
+}
+void launch_reorder(const splat_adray_t* src, const int32_t* hsrc, int64_t R, splat_adray_t* dst, int32_t* hdst,
+                    hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(k_reorder, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, src, hsrc, R, dst, hdst);
 }
 void launch_raygen(const splat_pinhole_t& cam, splat_adray_t* rays, int64_t R, hipStream_t s) {
   if (R <= 0) return;

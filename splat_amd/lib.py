@@ -62,6 +62,9 @@ SIGNATURES = {
     "splat_run_evolution": (i32, [P, i64, P, P, i32, P, P, P, sz, P, P]),
     "splat_trace_frame": (i32, [P, P, i64, P, P, i32, P, P, P, P, P, sz, P]),
     "splat_param_sizes": (i32, [P]),
+    "splat_reorder_ws_bytes": (sz, [i64]),
+    "splat_reorder_by_dest": (i32, [P, i64, P, P, sz, P]),
+    "splat_shade": (i32, [P, i64, P, P, i32, P, P, P, i32, i32, i64, P, P]),
     "splat_engine_frame_bytes": (sz, [P, i32]),
     "splat_engine_create": (i32, [i64, i32, P]),
     "splat_engine_destroy": (i32, [P]),

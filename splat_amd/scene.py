@@ -215,3 +215,22 @@ def scene_c3() -> Scene:
 def camera_c3(seed: int = 3, height: int = 2048, width: int = 2048, spp: int = 8, **kw) -> abi.Pinhole:
     return pinhole((0.3, 0.1, 7.0), (0.0, 0.0, -1.5), (0.0, 1.0, 0.0), 90.0, height, width, spp=spp,
                    lambda_mode=abi.LAMBDA_STRATIFIED, seed=seed, **kw)
+
+
+def view_camera(height: int, width: int, *, pos=(3.5, -4.0, 3.0), look_at=(0.0, 0.0, -0.5), fov_deg: float = 60.0,
+                seed: int = 11) -> abi.Pinhole:
+    """The viewer's camera of the backward pass (camera_ray at lambda 550 nm, src/tracer.jl:347-352,
+    src/ray_generators.jl:52-59): pixel order, one ray per pixel, seeded 0.025-pixel jitter."""
+    return pinhole(pos, look_at, (0.0, 0.0, 1.0), fov_deg, height, width, lambda_mode=abi.LAMBDA_FIXED,
+                   lambda_min=550.0, lambda_max=550.0, jitter=0.025, seed=seed, order=abi.ORDER_COLUMN_MAJOR)
+
+
+def skydome(width: int = 256, height: int = 128) -> np.ndarray:
+    """A procedural sky (3, height, width) float32: blue zenith to pale horizon along u (the polar
+    angle), a warm band around one azimuth along v."""
+    u = (np.arange(width, dtype=np.float64) + 0.5) / width
+    v = (np.arange(height, dtype=np.float64) + 0.5) / height
+    uu, vv = np.meshgrid(u, v)
+    band = np.exp(-((vv - 0.3) / 0.08) ** 2)
+    sky = np.stack([0.25 + 0.6 * uu + 0.3 * band, 0.45 + 0.4 * uu + 0.15 * band, 0.9 - 0.2 * uu])
+    return np.clip(sky, 0.0, 1.0).astype(np.float32)
