@@ -247,6 +247,43 @@ def served(n_requests: int = 120, rate: float = 40.0):
     return out
 
 
+def render_frame(frames: int = 20):
+    """One trace! iteration per step at 1 MP on the 10,240-triangle scene: forward light map from
+    1024^2 light rays (uniform wavelengths) into 13 spectral planes of 1024^2 (400-700 nm every
+    25 nm, the reference's grid), 4 bounces; backward image from 1024^2 camera rays, 4 bounces,
+    reorder, skydome + retina-weighted light-map shading. Both passes use the tree BV hitter."""
+    import numpy as np
+    import torch
+
+    from splat_amd import abi, api, bv, scene
+
+    sc = scene.scene_c2(seed=2)
+    tris = api.to_device(sc.ftris)
+    bvs, mem, n_top = bv.tree_partition(sc.ftris)
+    hitter = api.HIPBVHitter(tris, bvs, mem, n_top=n_top)
+    light = scene.camera_c2(seed=2, jitter=0.025)
+    cam = scene.view_camera(1024, 1024, pos=(3.5, -4.5, 3.0), look_at=(0.0, 0.0, -0.5), seed=12)
+    sky = torch.from_numpy(scene.skydome(512, 256)).cuda()
+    r = api.Renderer(forward_hitter=hitter, backward_hitter=hitter, cam=cam, light=light, tris=tris, depth=4,
+                     first_diffuse=sc.first_diffuse, spectrum=np.arange(400.0, 701.0, 25.0, dtype=np.float32),
+                     tex_width=1024, tex_height=1024, skydome=sky, seed=5)
+    for _ in range(3):
+        r.render()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(frames + 1)]
+    evs[0].record()
+    for i in range(frames):
+        r.render()
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    ms = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(frames)])
+    return {"workload": "trace! iteration: 1024^2 light rays -> 13 x 1024^2 spectral light map; 1024^2 camera rays "
+                        "-> reorder -> shaded RGB image; 4 bounces each, 10,243 triangles",
+            "frames": frames, "p50_ms": round(float(np.percentile(ms, 50)), 3),
+            "p99_ms": round(float(np.percentile(ms, 99)), 3),
+            "mrays_per_s": round(2 * (1 << 20) / float(np.mean(ms)) / 1e3, 1)}
+
+
 def served_1mp(n_requests: int = 300, rate: float = 400.0):
     """The latency case: Poisson arrivals of 1 MP requests (1024^2 rays, one seeded wavelength per
     ray, 4 bounces, 1024^2 RGB light map) over the 10,240-triangle two-icosphere scene, each with
@@ -402,6 +439,8 @@ def main() -> int:
     serve_out = None if args.no_served else served()
     torch.cuda.empty_cache()
     serve_1mp = None if args.no_served else served_1mp()
+    torch.cuda.empty_cache()
+    render = None if args.no_served else render_frame()
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, params, lm)
     small = None if args.no_cpu_baseline else small_frame()
     out = {
@@ -458,6 +497,7 @@ def main() -> int:
         "batched": batched,
         "served": serve_out,
         "served_1mp": serve_1mp,
+        "render": render,
         "cpu_baseline": cpu,
         "small_frame": small,
     }

@@ -253,27 +253,53 @@ def continuum_shade_(*, tracer: HIPTracer, tris, rays, tex, lm: abi.LightMapPara
     return rgb
 
 
-def trace_(*, forward_hitter, backward_hitter, cam: abi.Pinhole, light: abi.Pinhole, tris, depth: int,
-           first_diffuse: int, spectrum, tex_width: int, tex_height: int, skydome=None, seed: int = 0,
-           force_rand=None, upscale: int = 1):
-    """trace! (src/tracer.jl:291-385), one iteration: the forward light map -- rays from the
-    light pinhole, run_evolution!, continuum_light_map2! into one plane per spectrum entry --
-    then the backward image -- camera rays (one per (upscale x upscale) pixel block, lambda 550),
-    run_evolution!(reorder = true), continuum_shade! with upscale^2 sub-pixel offsets.
-    Returns (rgb (n_pixels, 3) float32, light map)."""
-    lm = abi.lightmap_params(tex_width, tex_height, spectrum=spectrum, mode="spectral")
-    fr_p = abi.run_params(depth, first_diffuse, math.nan if force_rand is None else float(force_rand), seed=seed)
-    fwd = Frame(light, forward_hitter, tris, fr_p, lm)
-    fwd.trace()
-    rays = rays_from_pinhole(cam)
-    tr = HIPTracer(rays, upscale)
-    run_evolution_(backward_hitter, tr, depth=depth, first_diffuse=first_diffuse, tris=tris, rays=rays,
-                   force_rand=force_rand, seed=seed + 1)
-    reorder_(rays, tr)
-    shade_lm = abi.lightmap_params(tex_width, tex_height, n_delta=upscale, spectrum=spectrum, mode="spectral")
-    rgb = continuum_shade_(tracer=tr, tris=tris, rays=rays, tex=fwd.tex, lm=shade_lm, grid_h=cam.height,
-                           skydome=skydome)
-    return rgb, fwd.tex
+class Renderer:
+    """trace! (src/tracer.jl:291-385) with its buffers kept between iterations: the forward light
+    map -- rays from the light pinhole, run_evolution!, continuum_light_map2! into one plane per
+    spectrum entry -- then the backward image -- camera rays (lambda 550, one per upscale x
+    upscale pixel block), run_evolution!(reorder = true), continuum_shade! with upscale^2 sub-pixel
+    offsets -- into `rgb` ((h*upscale) x (w*upscale) pixels x 3, row index fastest)."""
+
+    def __init__(self, *, forward_hitter, backward_hitter, cam: abi.Pinhole, light: abi.Pinhole, tris, depth: int,
+                 first_diffuse: int, spectrum, tex_width: int, tex_height: int, skydome=None, seed: int = 0,
+                 force_rand=None, upscale: int = 1):
+        torch = _torch()
+        self.cam, self.tris, self.depth, self.first_diffuse = cam, tris, depth, first_diffuse
+        self.backward_hitter, self.skydome, self.seed, self.force_rand = backward_hitter, skydome, seed, force_rand
+        lm = abi.lightmap_params(tex_width, tex_height, spectrum=spectrum, mode="spectral")
+        fr_p = abi.run_params(depth, first_diffuse, math.nan if force_rand is None else float(force_rand), seed=seed)
+        self.forward = Frame(light, forward_hitter, tris, fr_p, lm)
+        self.shade_lm = abi.lightmap_params(tex_width, tex_height, n_delta=upscale, spectrum=spectrum, mode="spectral")
+        R = cam.height * cam.width * cam.samples_per_pixel
+        self.rays = torch.empty((R, abi.ADRAY.itemsize), dtype=torch.uint8, device=tris.device)
+        self.tracer = HIPTracer(self.rays, upscale)
+        self.rgb = torch.empty((R * upscale * upscale, 3), dtype=torch.float32, device=tris.device)
+        self.reorder_ws = _ws(L.lib().splat_reorder_ws_bytes(R), tris.device)
+
+    @property
+    def tex(self):
+        return self.forward.tex
+
+    def render(self):
+        lib = L.lib()
+        s = L.stream_handle()
+        self.forward.trace()
+        R = _nrows(self.rays)
+        L.check(lib.splat_raygen_pinhole(L.ref(self.cam), L.ptr(self.rays), R, s), "raygen")
+        run_evolution_(self.backward_hitter, self.tracer, depth=self.depth, first_diffuse=self.first_diffuse,
+                       tris=self.tris, rays=self.rays, force_rand=self.force_rand, seed=self.seed + 1)
+        L.check(lib.splat_reorder_by_dest(L.ptr(self.rays), R, L.ptr(self.tracer.hit_idx), L.ptr(self.reorder_ws),
+                                          self.reorder_ws.numel(), s), "reorder")
+        continuum_shade_(tracer=self.tracer, tris=self.tris, rays=self.rays, tex=self.forward.tex, lm=self.shade_lm,
+                         grid_h=self.cam.height, skydome=self.skydome, rgb=self.rgb)
+        return self.rgb
+
+
+def trace_(**kw):
+    """One trace! iteration (Renderer.render); returns (rgb, light map)."""
+    r = Renderer(**kw)
+    rgb = r.render()
+    return rgb, r.tex
 
 
 class Frame:
