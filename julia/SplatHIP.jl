@@ -130,4 +130,28 @@ function run_evolution!(hitter::Union{HIPBruteForceHitter,HIPBVHitter}, tracer::
                 pointer(ws), length(ws), pointer(live), stream_ptr()), "run_evolution")
 end
 
+
+# run_evolution!(...; reorder = true)'s reorder (src/tracer.jl:270-278)
+function reorder!(tracer, rays::ROCArray{ADRay})
+    R = length(rays)
+    ws = ROCArray{UInt8}(undef, ccall((:splat_reorder_ws_bytes, LIB), Csize_t, (Int64,), R))
+    check(ccall((:splat_reorder_by_dest, LIB), Int32,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+                pointer(rays), R, pointer(tracer.hit_idx), pointer(ws), length(ws), stream_ptr()), "reorder")
+end
+
+# continuum_shade!(::ExperimentalImager2; ...) (src/ray_imagers.jl:440-503): lm holds the spectrum,
+# the sub-pixel offsets and (weights) the retina factors; RGB3 is (3, n_pixels) Float32
+function continuum_shade!(tracer, rays::ROCArray{ADRay}, tris::ROCArray{FTri}, lm::Ref, tex::ROCArray{Float32},
+                          skydome::Union{Nothing,ROCArray{Float32,3}}, grid_h, RGB3::ROCArray{Float32})
+    sky = skydome === nothing ? C_NULL : pointer(skydome)
+    sw = skydome === nothing ? 0 : size(skydome, 1)
+    sh = skydome === nothing ? 0 : size(skydome, 2)
+    check(ccall((:splat_shade, LIB), Int32,
+                (Ptr{Cvoid}, Int64, Ptr{Int32}, Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}, Int32, Int32,
+                 Int64, Ptr{Float32}, Ptr{Cvoid}),
+                pointer(rays), length(rays), pointer(tracer.hit_idx), pointer(tris), length(tris), lm, pointer(tex),
+                sky, sw, sh, grid_h, pointer(RGB3), stream_ptr()), "shade")
+end
+
 end # module

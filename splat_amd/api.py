@@ -258,7 +258,11 @@ class Renderer:
     map -- rays from the light pinhole, run_evolution!, continuum_light_map2! into one plane per
     spectrum entry -- then the backward image -- camera rays (lambda 550, one per upscale x
     upscale pixel block), run_evolution!(reorder = true), continuum_shade! with upscale^2 sub-pixel
-    offsets -- into `rgb` ((h*upscale) x (w*upscale) pixels x 3, row index fastest)."""
+    offsets -- into `rgb` ((h*upscale) x (w*upscale) pixels x 3, row index fastest). The two
+    passes are independent until shading, so the backward bounce loop runs on a second stream
+    beside the forward pass (the reference runs the forward pass as an @async task,
+    src/tracer.jl:317); iterations > 1 average several seeded iterations into `out`
+    (iterations_per_frame, src/tracer.jl:311, :380-384)."""
 
     def __init__(self, *, forward_hitter, backward_hitter, cam: abi.Pinhole, light: abi.Pinhole, tris, depth: int,
                  first_diffuse: int, spectrum, tex_width: int, tex_height: int, skydome=None, seed: int = 0,
@@ -274,25 +278,49 @@ class Renderer:
         self.rays = torch.empty((R, abi.ADRAY.itemsize), dtype=torch.uint8, device=tris.device)
         self.tracer = HIPTracer(self.rays, upscale)
         self.rgb = torch.empty((R * upscale * upscale, 3), dtype=torch.float32, device=tris.device)
+        self.out = torch.empty_like(self.rgb)
         self.reorder_ws = _ws(L.lib().splat_reorder_ws_bytes(R), tris.device)
+        self.side = torch.cuda.Stream(device=tris.device)  # default priority
 
     @property
     def tex(self):
         return self.forward.tex
 
-    def render(self):
+    def _iteration(self, it: int):
+        torch = _torch()
         lib = L.lib()
-        s = L.stream_handle()
-        self.forward.trace()
+        main = torch.cuda.current_stream()
         R = _nrows(self.rays)
-        L.check(lib.splat_raygen_pinhole(L.ref(self.cam), L.ptr(self.rays), R, s), "raygen")
-        run_evolution_(self.backward_hitter, self.tracer, depth=self.depth, first_diffuse=self.first_diffuse,
-                       tris=self.tris, rays=self.rays, force_rand=self.force_rand, seed=self.seed + 1)
-        L.check(lib.splat_reorder_by_dest(L.ptr(self.rays), R, L.ptr(self.tracer.hit_idx), L.ptr(self.reorder_ws),
-                                          self.reorder_ws.numel(), s), "reorder")
+        self.side.wait_stream(main)
+        with torch.cuda.stream(self.side):  # backward bounce loop beside the forward pass
+            s = L.stream_handle(self.side)
+            self.cam.seed = self.cam_seed + it
+            L.check(lib.splat_raygen_pinhole(L.ref(self.cam), L.ptr(self.rays), R, s), "raygen")
+            run_evolution_(self.backward_hitter, self.tracer, depth=self.depth, first_diffuse=self.first_diffuse,
+                           tris=self.tris, rays=self.rays, force_rand=self.force_rand, seed=self.seed + 2 * it + 1)
+            L.check(lib.splat_reorder_by_dest(L.ptr(self.rays), R, L.ptr(self.tracer.hit_idx),
+                                              L.ptr(self.reorder_ws), self.reorder_ws.numel(), s), "reorder")
+        self.forward.params.seed = self.seed + 2 * it
+        self.forward.cam.seed = self.light_seed + it
+        self.forward.trace()
+        main.wait_stream(self.side)
         continuum_shade_(tracer=self.tracer, tris=self.tris, rays=self.rays, tex=self.forward.tex, lm=self.shade_lm,
                          grid_h=self.cam.height, skydome=self.skydome, rgb=self.rgb)
-        return self.rgb
+
+    def render(self, iterations: int = 1):
+        """One frame: `iterations` seeded iterations averaged (out <- out (f-1)/f + rgb/f)."""
+        self.cam_seed = getattr(self, "cam_seed", self.cam.seed)
+        self.light_seed = getattr(self, "light_seed", self.forward.cam.seed)
+        if iterations == 1:
+            self._iteration(0)
+            return self.rgb
+        for f in range(1, iterations + 1):
+            self._iteration(f - 1)
+            if f == 1:
+                self.out.copy_(self.rgb)
+            else:
+                self.out.mul_((f - 1) / f).add_(self.rgb, alpha=1.0 / f)
+        return self.out
 
 
 def trace_(**kw):

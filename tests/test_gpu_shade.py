@@ -101,3 +101,30 @@ def test_trace_iteration_vs_oracle():
     got = rgb.cpu().numpy()
     assert want_rgb.max() > 0.05
     assert np.abs(got - want_rgb).max() < 1e-4  # the light map's atomic order, through bilinear reads
+
+
+def test_render_accumulates_iterations():
+    """iterations_per_frame (src/tracer.jl:311, :380-384): the frame is the running average of
+    the seeded iterations, out <- out (f-1)/f + rgb/f."""
+    import torch
+
+    sc = scene.scene_c1()
+    tris = api.to_device(sc.ftris)
+    hitter = api.HIPBruteForceHitter(tris)
+    kw = dict(forward_hitter=hitter, backward_hitter=hitter,
+              cam=scene.view_camera(24, 32, pos=(2.5, -3.0, 2.5), look_at=(0.0, 0.0, -0.5), seed=4),
+              light=scene.camera_c1(seed=3, height=64, width=64), tris=tris, depth=2, first_diffuse=sc.first_diffuse,
+              spectrum=np.arange(400.0, 701.0, 50.0, dtype=np.float32), tex_width=64, tex_height=64,
+              skydome=torch.from_numpy(scene.skydome(32, 16)).cuda(), seed=8)
+    r = api.Renderer(**kw)
+    frame = r.render(iterations=3).clone()
+    singles = []
+    for it in range(3):
+        r._iteration(it)
+        singles.append(r.rgb.clone())
+    _sync()
+    want = singles[0]
+    for f in (2, 3):
+        want = want * ((f - 1) / f) + singles[f - 1] * (1.0 / f)
+    assert torch.allclose(frame, want, atol=1e-6)
+    assert not torch.equal(singles[0], singles[1])  # the iterations differ (seeds)
