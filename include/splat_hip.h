@@ -185,6 +185,28 @@ typedef struct splat_pinhole {
   uint64_t seed;
 } splat_pinhole_t;
 
+/* ---- rectangular light sources (RectLight, rays_from_light{,s}, src/ray_generators.jl:4-13,
+ * :64-113) ---------------------------------------------------------------------------------- */
+#define SPLAT_MAX_LIGHTS 16
+
+typedef struct splat_rect_light {
+  float center[3]; /* centre of the emitting rectangle                                  */
+  float dir[3];    /* emission direction, used as given                                 */
+  float dim1[3];   /* half extent along rows (x)                                        */
+  float dim2[3];   /* half extent along columns (y)                                     */
+  int64_t res1;    /* rows before upscale                                               */
+  int64_t res2;    /* columns before upscale                                            */
+} splat_rect_light_t; /* 64 bytes: Julia's isbits RectLight */
+
+typedef struct splat_light_rays {
+  int32_t upscale;     /* forward_upscale: a light emits (res1/upscale) x (res2/upscale) rays */
+  int32_t lambda_mode; /* SPLAT_LAMBDA_FIXED (the reference: 550 nm) or SPLAT_LAMBDA_UNIFORM  */
+  float lambda_min, lambda_max;
+  float jitter;        /* cell offsets dx, dy = jitter * U[0,1) (the reference: rand(Float32)) */
+  int32_t _pad;
+  uint64_t seed;
+} splat_light_rays_t;
+
 /* ---- fused bounce loop (run_evolution!, src/tracer.jl:233-282) --------------------------- */
 typedef struct splat_run_params {
   int32_t depth;         /* bounces                                                    */
@@ -286,6 +308,18 @@ int32_t splat_light_map(splat_adray_t* rays, int64_t R, const int64_t* d_count,
 int32_t splat_raygen_pinhole(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R,
                              void* stream);
 
+/* rays_from_lights(lights, upscale) (src/ray_generators.jl:64-113). `lights` is host memory
+ * (a Julia Vector{RectLight}). Light i's rays follow light i-1's; ray j of a light is cell
+ * (j % h + 1, j / h + 1) of its h x w = (res1/upscale) x (res2/upscale) grid, column-major, at
+ * offset (x + dx, y + dy) with origin center + dim1 (x+dx - h/2)/(h/2) + dim2 (y+dy - w/2)/(w/2)
+ * (integer halves), dest 0. With several lights each must emit res1*res2/upscale^2 rays, the
+ * slice the reference sizes for it (src/ray_generators.jl:107). Returns the total ray count, or
+ * a negative error code. */
+int64_t splat_lights_ray_count(const splat_rect_light_t* lights, int32_t n_lights, int32_t upscale);
+int32_t splat_raygen_lights(const splat_rect_light_t* lights, int32_t n_lights,
+                            const splat_light_rays_t* params, splat_adray_t* rays, int64_t R,
+                            void* stream);
+
 /* Fill helper (graph-safe clear, no hipMemsetAsync): n floats <- value. */
 int32_t splat_fill_f32(float* dst, int64_t n, float value, void* stream);
 
@@ -308,6 +342,16 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
                           int32_t n_tris_len, int32_t* hit_idx, const splat_run_params_t* params,
                           const splat_lightmap_params_t* lm, float* tex, void* ws,
                           size_t ws_bytes, void* stream);
+
+/* The forward pass of trace! (src/tracer.jl:316-341): rays_from_lights(lights, upscale) ->
+ * run_evolution! -> continuum_light_map2!, all on the device, the splat inside the bounce loop.
+ * R = splat_lights_ray_count(lights, n_lights, lp->upscale); the rest as splat_trace_frame. */
+int32_t splat_trace_lights(const splat_rect_light_t* lights, int32_t n_lights,
+                           const splat_light_rays_t* lp, splat_adray_t* rays, int64_t R,
+                           const splat_hitset_t* hs, const splat_ftri_t* tris, int32_t n_tris_len,
+                           int32_t* hit_idx, const splat_run_params_t* params,
+                           const splat_lightmap_params_t* lm, float* tex, void* ws, size_t ws_bytes,
+                           void* stream);
 
 /* ---- backward pass: camera rays -> image (the rest of trace!, src/tracer.jl:346-385) ------ */
 /* run_evolution!(...; reorder = true)'s reorder (src/tracer.jl:270-278): ray r and its hit index

@@ -154,4 +154,26 @@ function continuum_shade!(tracer, rays::ROCArray{ADRay}, tris::ROCArray{FTri}, l
                 sky, sw, sh, grid_h, pointer(RGB3), stream_ptr()), "shade")
 end
 
+# rays_from_lights(lights, upscale) (src/ray_generators.jl:64-113) on the device. The reference's
+# Vector{RectLight} is passed as is (64-byte isbits records); rand(Float32)'s cell offsets come
+# from `seed` (lambda_mode 0 = every ray at lambda_min, 1 = uniform in [lambda_min, lambda_max)).
+struct LightRays
+    upscale::Int32; lambda_mode::Int32; lambda_min::Float32; lambda_max::Float32
+    jitter::Float32; _pad::Int32
+    seed::UInt64
+end
+LightRays(upscale; lambda_mode = 0, lambda_min = 550f0, lambda_max = 550f0, jitter = 1f0, seed = 0) =
+    LightRays(upscale, lambda_mode, lambda_min, lambda_max, jitter, 0, seed)
+
+function rays_from_lights(lights::Vector{RectLight}, upscale; kw...)
+    p = LightRays(upscale; kw...)
+    R = ccall((:splat_lights_ray_count, LIB), Int64, (Ptr{RectLight}, Int32, Int32), lights, length(lights), upscale)
+    R < 0 && throw(SplatError(Int32(R), "rays_from_lights"))
+    rays = ROCArray{ADRay}(undef, R)
+    check(ccall((:splat_raygen_lights, LIB), Int32,
+                (Ptr{RectLight}, Int32, Ref{LightRays}, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+                lights, length(lights), p, pointer(rays), R, stream_ptr()), "rays_from_lights")
+    return rays
+end
+
 end # module

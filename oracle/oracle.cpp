@@ -1063,6 +1063,56 @@ void oracle_raygen(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R) {
   for (int64_t q = 0; q < R; ++q) raygen_one(*cam, q, &rays[q]);
 }
 
+// rays_from_lights(lights, upscale) (src/ray_generators.jl:64-113): each light's rays in turn;
+// within a light, ray_generator.(1:h, (1:w)') reshaped to a vector is column-major, rows fastest.
+// rect_light_ray (:66-94) in Float32, in the reference's order: origin(x, y) = center +
+// dim1 * (x - h/2) / (h/2) + dim2 * (y - w/2) / (w/2) (Julia evaluates a * b / c as (a * b) / c
+// and a + b + c as (a + b) + c; h/2 is integer division), its ForwardDiff derivatives in x and y
+// (dual partial dim1 * 1 divided by h/2), dest 0 and (x, y) = (x + dx, y + dy). rand(Float32)'s
+// offsets are replaced by seeded ones: dx = jitter * u01(seed, 4<<32 | 2 light, j), dy from
+// stream 2 light + 1; a uniform wavelength from stream RNG_LAMBDA | (light + 1).
+// Returns the number of rays written.
+int64_t oracle_raygen_lights(const splat_rect_light_t* lights, int32_t n_lights, const splat_light_rays_t* p,
+                             splat_adray_t* rays) {
+  const uint64_t RNG_LIGHT = 4ull << 32;
+  int64_t out = 0;
+  for (int32_t li = 0; li < n_lights; ++li) {
+    const splat_rect_light_t& L = lights[li];
+    const int64_t h = L.res1 / p->upscale, w = L.res2 / p->upscale;
+    const float half_h = (float)(h / 2), half_w = (float)(w / 2);
+    int64_t j = 0;
+    for (int64_t y = 1; y <= w; ++y) {
+      for (int64_t x = 1; x <= h; ++x, ++j) {
+        const float ox = p->jitter * u01(p->seed, RNG_LIGHT | (uint64_t)(2 * li), (uint64_t)j);
+        const float oy = p->jitter * u01(p->seed, RNG_LIGHT | (uint64_t)(2 * li + 1), (uint64_t)j);
+        const float xf = (float)x + ox, yf = (float)y + oy;
+        splat_adray_t r;
+        std::memset(&r, 0, sizeof r);
+        for (int k = 0; k < 3; ++k) {
+          const float a = (L.dim1[k] * (xf - half_h)) / half_h;
+          const float b = (L.dim2[k] * (yf - half_w)) / half_w;
+          r.pos[k] = (L.center[k] + a) + b;
+          r.dpdx[k] = (L.dim1[k] * 1.0f) / half_h;
+          r.dpdy[k] = (L.dim2[k] * 1.0f) / half_w;
+          r.dir[k] = L.dir[k];
+        }
+        r.in_medium = 0;
+        r.ignore_tri = 1;
+        r.dest = 0;
+        r.lambda = p->lambda_min;
+        if (p->lambda_mode == SPLAT_LAMBDA_UNIFORM)
+          r.lambda = std::fmaf(u01(p->seed, RNG_LAMBDA | (uint64_t)(li + 1), (uint64_t)j),
+                               p->lambda_max - p->lambda_min, p->lambda_min);
+        r.x = xf;
+        r.y = yf;
+        r.status = SPLAT_RAY_ACTIVE;
+        rays[out++] = r;
+      }
+    }
+  }
+  return out;
+}
+
 // run_evolution! with the brute-force (bv_counts == NULL) or the BV-culled hitter
 int64_t oracle_run_evolution_f32(splat_adray_t* rays, int64_t R, const splat_ftri_t* tris, int32_t N, int32_t* hit_idx,
                                  const splat_run_params_t* params, int32_t n_bv, const int64_t* bv_counts,

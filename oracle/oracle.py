@@ -68,6 +68,7 @@ def lib():
             "oracle_light_map_f32": (None, [P, i64, P, P, i32, P, P]),
             "oracle_light_map_f64": (None, [P, i64, P, P, i32, P, P]),
             "oracle_raygen": (None, [P, P, i64]),
+            "oracle_raygen_lights": (i64, [P, i32, P, P]),
             "oracle_run_evolution_f32": (i64, [P, i64, P, i32, P, P, i32, P, P, i64, i32]),
             "oracle_trace_frame_f32": (i64, [P, P, i64, P, i32, P, P, P, P, i32, P, P, i64, i32]),
             "oracle_trace_sample_f32": (i64, [P, i64, i64, i64, P, i32, P, P, P, i32, P, P, i64, i32]),
@@ -184,6 +185,17 @@ def raygen(cam):
     R = cam.height * cam.width * cam.samples_per_pixel
     rays = np.zeros(R, dtype=abi.ADRAY)
     lib().oracle_raygen(_ref(cam), _p(rays), R)
+    return rays
+
+
+def raygen_lights(lights, params):
+    """rays_from_lights over a sequence of abi.RectLight with abi.LightRays parameters."""
+    arr = (abi.RectLight * max(len(lights), 1))(*lights)
+    up = params.upscale
+    R = sum((L.res1 // up) * (L.res2 // up) for L in lights)
+    rays = np.zeros(R, dtype=abi.ADRAY)
+    n = lib().oracle_raygen_lights(arr, len(lights), _ref(params), _p(rays))
+    assert n == R
     return rays
 
 

@@ -88,6 +88,38 @@ class Pinhole(C.Structure):
     ]
 
 
+MAX_LIGHTS = 16
+
+
+class RectLight(C.Structure):
+    """RectLight (src/ray_generators.jl:4-11): four SVector{3,Float32} then two Int, 64 bytes."""
+    _fields_ = [
+        ("center", C.c_float * 3), ("dir", C.c_float * 3), ("dim1", C.c_float * 3), ("dim2", C.c_float * 3),
+        ("res1", C.c_int64), ("res2", C.c_int64),
+    ]
+
+
+class LightRays(C.Structure):
+    _fields_ = [
+        ("upscale", C.c_int32), ("lambda_mode", C.c_int32), ("lambda_min", C.c_float), ("lambda_max", C.c_float),
+        ("jitter", C.c_float), ("_pad", C.c_int32), ("seed", C.c_uint64),
+    ]
+
+
+def rect_light(center, dir, dim1, dim2, res1: int, res2: int) -> RectLight:
+    L = RectLight()
+    for k in range(3):
+        L.center[k], L.dir[k], L.dim1[k], L.dim2[k] = center[k], dir[k], dim1[k], dim2[k]
+    L.res1, L.res2 = res1, res2
+    return L
+
+
+def light_rays(upscale: int = 1, *, lambda_mode: int = LAMBDA_FIXED, lambda_min: float = 550.0,
+               lambda_max: float = 550.0, jitter: float = 1.0, seed: int = 0) -> LightRays:
+    """Defaults follow rays_from_lights: 550 nm, cell offsets U[0, 1) (src/ray_generators.jl:59, :75-76)."""
+    return LightRays(upscale, lambda_mode, lambda_min, lambda_max, jitter, 0, seed)
+
+
 class RunParams(C.Structure):
     _fields_ = [
         ("depth", C.c_int32), ("first_diffuse", C.c_int32), ("force_rand", C.c_float), ("_pad", C.c_int32),
@@ -107,6 +139,7 @@ class FrameRequest(C.Structure):
 
 PARAM_STRUCTS = [Optics, HitSet, LightMapParams, Pinhole, RunParams, FrameRequest]
 assert C.sizeof(Pinhole) == 96 and C.sizeof(RunParams) == 56 and C.sizeof(HitSet) == 88
+assert C.sizeof(RectLight) == 64 and C.sizeof(LightRays) == 32
 
 
 def optics(model: int = IOR_CAUCHY, a: float = 1.5, b: float = 0.004, compat: int = COMPAT_REFERENCE) -> Optics:

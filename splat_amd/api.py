@@ -221,6 +221,23 @@ def rays_from_pinhole(cam: abi.Pinhole, device: str = "cuda"):
     return rays
 
 
+def rays_from_lights(lights, params: abi.LightRays | None = None, device: str = "cuda"):
+    """rays_from_lights(lights, upscale) (src/ray_generators.jl:64-113) on the device: `lights` is
+    a sequence of abi.RectLight; each emits (res1/upscale) x (res2/upscale) rays, column-major,
+    light after light. The reference's unseeded rand(Float32) cell offsets come from `params.seed`."""
+    torch = _torch()
+    L.require_gpu()
+    params = params if params is not None else abi.light_rays()
+    arr = (abi.RectLight * max(len(lights), 1))(*lights)
+    R = L.lib().splat_lights_ray_count(arr, len(lights), params.upscale)
+    if R < 0:
+        L.check(int(R), "rays_from_lights")
+    rays = torch.empty((R, abi.ADRAY.itemsize), dtype=torch.uint8, device=device)
+    L.check(L.lib().splat_raygen_lights(arr, len(lights), L.ref(params), L.ptr(rays), R, L.stream_handle()),
+            "rays_from_lights")
+    return rays
+
+
 def reorder_(rays, tracer: HIPTracer) -> None:
     """run_evolution!(...; reorder = true)'s reorder (src/tracer.jl:270-278): rays and
     tracer.hit_idx back to pixel order (slot dest - 1)."""
@@ -264,15 +281,27 @@ class Renderer:
     src/tracer.jl:317); iterations > 1 average several seeded iterations into `out`
     (iterations_per_frame, src/tracer.jl:311, :380-384)."""
 
-    def __init__(self, *, forward_hitter, backward_hitter, cam: abi.Pinhole, light: abi.Pinhole, tris, depth: int,
-                 first_diffuse: int, spectrum, tex_width: int, tex_height: int, skydome=None, seed: int = 0,
-                 force_rand=None, upscale: int = 1):
+    def __init__(self, *, forward_hitter, backward_hitter, cam: abi.Pinhole, tris, depth: int, first_diffuse: int,
+                 spectrum, tex_width: int, tex_height: int, light: abi.Pinhole | None = None, lights=None,
+                 light_params: abi.LightRays | None = None, skydome=None, seed: int = 0, force_rand=None,
+                 upscale: int = 1):
+        """The forward rays come from `lights` (abi.RectLight list, rays_from_lights with
+        `light_params`, as trace! does) or from the point-light pinhole `light`."""
         torch = _torch()
         self.cam, self.tris, self.depth, self.first_diffuse = cam, tris, depth, first_diffuse
         self.backward_hitter, self.skydome, self.seed, self.force_rand = backward_hitter, skydome, seed, force_rand
-        lm = abi.lightmap_params(tex_width, tex_height, spectrum=spectrum, mode="spectral")
         fr_p = abi.run_params(depth, first_diffuse, math.nan if force_rand is None else float(force_rand), seed=seed)
-        self.forward = Frame(light, forward_hitter, tris, fr_p, lm)
+        if (light is None) == (lights is None):
+            raise ValueError("give exactly one forward ray source: light (pinhole) or lights (RectLights)")
+        if lights is not None:
+            lp = light_params if light_params is not None else abi.light_rays()
+            lm = abi.lightmap_params(tex_width, tex_height, n_delta=lp.upscale, spectrum=spectrum, mode="spectral")
+            self.forward = LightFrame(lights, lp, forward_hitter, tris, fr_p, lm, device=tris.device)
+            self.forward_source = lp
+        else:
+            lm = abi.lightmap_params(tex_width, tex_height, spectrum=spectrum, mode="spectral")
+            self.forward = Frame(light, forward_hitter, tris, fr_p, lm, device=tris.device)
+            self.forward_source = light
         self.shade_lm = abi.lightmap_params(tex_width, tex_height, n_delta=upscale, spectrum=spectrum, mode="spectral")
         R = cam.height * cam.width * cam.samples_per_pixel
         self.rays = torch.empty((R, abi.ADRAY.itemsize), dtype=torch.uint8, device=tris.device)
@@ -301,7 +330,7 @@ class Renderer:
             L.check(lib.splat_reorder_by_dest(L.ptr(self.rays), R, L.ptr(self.tracer.hit_idx),
                                               L.ptr(self.reorder_ws), self.reorder_ws.numel(), s), "reorder")
         self.forward.params.seed = self.seed + 2 * it
-        self.forward.cam.seed = self.light_seed + it
+        self.forward_source.seed = self.light_seed + it
         self.forward.trace()
         main.wait_stream(self.side)
         continuum_shade_(tracer=self.tracer, tris=self.tris, rays=self.rays, tex=self.forward.tex, lm=self.shade_lm,
@@ -310,7 +339,7 @@ class Renderer:
     def render(self, iterations: int = 1):
         """One frame: `iterations` seeded iterations averaged (out <- out (f-1)/f + rgb/f)."""
         self.cam_seed = getattr(self, "cam_seed", self.cam.seed)
-        self.light_seed = getattr(self, "light_seed", self.forward.cam.seed)
+        self.light_seed = getattr(self, "light_seed", self.forward_source.seed)
         if iterations == 1:
             self._iteration(0)
             return self.rgb
@@ -350,6 +379,32 @@ class Frame:
                                           L.ptr(self.tris), _nrows(self.tris), L.ptr(self.hit_idx), L.ref(self.params),
                                           L.ref(self.lm), L.ptr(self.tex), L.ptr(self.ws), self.ws.numel(),
                                           L.stream_handle(stream)), "trace_frame")
+
+
+class LightFrame:
+    """The forward pass of trace! from rectangular lights (src/tracer.jl:316-341): rays_from_lights
+    -> run_evolution! -> continuum_light_map2!, one call, no host sync."""
+
+    def __init__(self, lights, lp: abi.LightRays, hitter, tris, params: abi.RunParams, lm: abi.LightMapParams,
+                 device: str = "cuda"):
+        torch = _torch()
+        L.require_gpu()
+        self.lights = (abi.RectLight * max(len(lights), 1))(*lights)
+        self.n_lights, self.lp = len(lights), lp
+        self.hitter, self.tris, self.params, self.lm = hitter, tris, params, lm
+        self.R = int(L.lib().splat_lights_ray_count(self.lights, self.n_lights, lp.upscale))
+        if self.R < 0:
+            L.check(self.R, "rays_from_lights")
+        self.rays = torch.empty((self.R, abi.ADRAY.itemsize), dtype=torch.uint8, device=device)
+        self.hit_idx = torch.empty(self.R, dtype=torch.int32, device=device)
+        self.tex = torch.zeros((lm.channels, lm.height, lm.width), dtype=torch.float32, device=device)
+        self.ws = _ws(L.lib().splat_run_ws_bytes(self.R), device)
+
+    def trace(self, stream=None) -> None:
+        L.check(L.lib().splat_trace_lights(self.lights, self.n_lights, L.ref(self.lp), L.ptr(self.rays), self.R,
+                                           L.ref(self.hitter.hs), L.ptr(self.tris), _nrows(self.tris),
+                                           L.ptr(self.hit_idx), L.ref(self.params), L.ref(self.lm), L.ptr(self.tex),
+                                           L.ptr(self.ws), self.ws.numel(), L.stream_handle(stream)), "trace_lights")
 
 
 class Engine:

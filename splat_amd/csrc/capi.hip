@@ -459,6 +459,42 @@ int32_t splat_raygen_pinhole(const splat_pinhole_t* cam, splat_adray_t* rays, in
   return launch_status();
 }
 
+static_assert(sizeof(splat_rect_light_t) == 64, "RectLight is 64 bytes");
+static_assert(sizeof(splat_light_rays_t) == 32, "light ray parameters are 32 bytes");
+
+int64_t splat_lights_ray_count(const splat_rect_light_t* lights, int32_t n, int32_t upscale) {
+  if (n < 0 || n > SPLAT_MAX_LIGHTS || (n > 0 && !lights) || upscale < 1) return SPLAT_ERR_INVALID_ARG;
+  int64_t R = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const int64_t r1 = lights[i].res1, r2 = lights[i].res2;
+    if (r1 < 0 || r2 < 0 || r1 > (1 << 20) || r2 > (1 << 20)) return SPLAT_ERR_INVALID_ARG;
+    const int64_t c = (r1 / upscale) * (r2 / upscale);
+    // several lights: light i fills a slice of res1 * res2 / upscale^2 rays (src/ray_generators.jl:107)
+    if (n > 1 && c != (r1 * r2) / ((int64_t)upscale * upscale)) return SPLAT_ERR_INVALID_ARG;
+    R += c;
+  }
+  return R;
+}
+
+int32_t splat_raygen_lights(const splat_rect_light_t* lights, int32_t n, const splat_light_rays_t* p,
+                            splat_adray_t* rays, int64_t R, void* stream) {
+  if (!p) return SPLAT_ERR_INVALID_ARG;
+  const int64_t total = splat_lights_ray_count(lights, n, p->upscale);
+  if (total < 0) return (int32_t)total;
+  if (total != R || (R > 0 && !rays)) return SPLAT_ERR_INVALID_ARG;
+  if (p->lambda_mode != SPLAT_LAMBDA_FIXED && p->lambda_mode != SPLAT_LAMBDA_UNIFORM) return SPLAT_ERR_INVALID_ARG;
+  LightSet A{};
+  A.n = n;
+  A.p = *p;
+  A.base[0] = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    A.light[i] = lights[i];
+    A.base[i + 1] = A.base[i] + (lights[i].res1 / p->upscale) * (lights[i].res2 / p->upscale);
+  }
+  launch_raygen_lights(A, rays, R, S(stream));
+  return launch_status();
+}
+
 int32_t splat_fill_f32(float* dst, int64_t n, float value, void* stream) {
   if (n < 0 || (n > 0 && !dst)) return SPLAT_ERR_INVALID_ARG;
   launch_fill_f32(dst, n, value, S(stream));
@@ -497,6 +533,25 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
   if (p->depth == 0) launch_raygen(*cam, w.s1, R, s);
   // the light map is splatted inside the loop, as rays become DIFFUSE
   if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s, lm, tex, p->depth > 0 ? cam : nullptr)) return e;
+  return launch_status();
+}
+
+int32_t splat_trace_lights(const splat_rect_light_t* lights, int32_t n, const splat_light_rays_t* lp,
+                           splat_adray_t* rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris,
+                           int32_t N, int32_t* hit_idx, const splat_run_params_t* p,
+                           const splat_lightmap_params_t* lm, float* tex, void* ws, size_t ws_bytes, void* stream) {
+  if (!lp || !tex) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes)) return e;
+  if (int32_t e = check_lightmap(lm)) return e;
+  const int64_t total = splat_lights_ray_count(lights, n, lp->upscale);
+  if (total < 0) return (int32_t)total;
+  if (total != R) return SPLAT_ERR_INVALID_ARG;
+  RunWs w = carve_run(ws, R);
+  hipStream_t s = S(stream);
+  // the rays go where the bounce loop starts; the light map is cleared by run_core and splatted
+  // inside the loop, as rays become DIFFUSE
+  if (int32_t e = splat_raygen_lights(lights, n, lp, w.s1, R, stream)) return e;
+  if (int32_t e = run_core(rays, R, hs, tris, hit_idx, p, w, s, lm, tex)) return e;
   return launch_status();
 }
 

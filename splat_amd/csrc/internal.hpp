@@ -104,6 +104,8 @@ void launch_shade(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const 
 void launch_reorder(const splat_adray_t* src, const int32_t* hsrc, int64_t R, splat_adray_t* dst, int32_t* hdst,
                     hipStream_t s);
 void launch_raygen(const splat_pinhole_t& cam, splat_adray_t* rays, int64_t R, hipStream_t s);
+struct LightSet;
+void launch_raygen_lights(const LightSet& A, splat_adray_t* rays, int64_t R, hipStream_t s);
 void launch_fill_f32(float* dst, int64_t n, float v, hipStream_t s);
 void launch_set_i64(int64_t* dst, int64_t v, hipStream_t s);
 void launch_copy_rays(const splat_adray_t* src, splat_adray_t* dst, int64_t R, const int64_t* d_count, hipStream_t s);

@@ -381,4 +381,54 @@ __device__ __forceinline__ void raygen_one(const splat_pinhole_t& cam, int64_t q
   *out = raygen_ray(cam, q);
 }
 
+// The lights of one rays_from_lights call, passed by value: light i owns rays [base[i], base[i+1]).
+struct LightSet {
+  splat_rect_light_t light[SPLAT_MAX_LIGHTS];
+  int64_t base[SPLAT_MAX_LIGHTS + 1];
+  splat_light_rays_t p;
+  int32_t n;
+};
+constexpr uint64_t RNG_LIGHT = 4ull << 32;  // cell offsets: stream RNG_LIGHT | 2 light (+1 for y)
+
+// rect_light_ray (src/ray_generators.jl:66-94) for ray q of the set: Float32 in the reference's
+// operation order, origin = (center + (dim1 * sx) / h2) + (dim2 * sy) / w2 with sx = (x + dx) - h2,
+// and its ForwardDiff tangents d origin / dx = dim1 / h2, d origin / dy = dim2 / w2 (h2 = h / 2 and
+// w2 = w / 2 are integer halves; h = 1 divides by zero exactly as the reference does).
+__device__ __forceinline__ splat_adray_t light_ray(const LightSet& A, int64_t q) {
+  int i = 0;
+  while (i + 1 < A.n && q >= A.base[i + 1]) ++i;
+  const splat_rect_light_t& L = A.light[i];
+  const int64_t j = q - A.base[i];
+  const int64_t h = L.res1 / A.p.upscale, w = L.res2 / A.p.upscale;
+  const int64_t x = j % h + 1, y = j / h + 1;
+  const float dx = A.p.jitter * u01(A.p.seed, RNG_LIGHT | (uint64_t)(2 * i), (uint64_t)j);
+  const float dy = A.p.jitter * u01(A.p.seed, RNG_LIGHT | (uint64_t)(2 * i + 1), (uint64_t)j);
+  const float X = (float)x + dx, Y = (float)y + dy;
+  const float h2 = (float)(h / 2), w2 = (float)(w / 2);
+  const float sx = X - h2, sy = Y - w2;
+  float lam = A.p.lambda_min;
+  if (A.p.lambda_mode == SPLAT_LAMBDA_UNIFORM)
+    lam = FMA(u01(A.p.seed, RNG_LAMBDA | (uint64_t)(i + 1), (uint64_t)j), A.p.lambda_max - A.p.lambda_min,
+              A.p.lambda_min);
+  splat_adray_t o;
+  for (int k = 0; k < 3; ++k) {
+    o.pos[k] = (L.center[k] + (L.dim1[k] * sx) / h2) + (L.dim2[k] * sy) / w2;
+    o.dpdl[k] = 0.f;
+    o.dpdx[k] = L.dim1[k] / h2;
+    o.dpdy[k] = L.dim2[k] / w2;
+    o.dir[k] = L.dir[k];
+    o.dddl[k] = 0.f; o.dddx[k] = 0.f; o.dddy[k] = 0.f;
+  }
+  o.in_medium = 0;
+  for (int k = 0; k < 7; ++k) o._pad0[k] = 0;
+  o.ignore_tri = 1;
+  o.dest = 0;  // forward rays: the pixel is not known ahead of time
+  o.lambda = lam;
+  o.x = X;
+  o.y = Y;
+  o.status = SPLAT_RAY_ACTIVE;
+  o._pad1[0] = o._pad1[1] = o._pad1[2] = 0;
+  return o;
+}
+
 }  // namespace spl

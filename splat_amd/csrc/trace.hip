@@ -123,6 +123,13 @@ __global__ __launch_bounds__(256) void k_raygen(const splat_pinhole_t cam, splat
   raygen_one(cam, q, rays + q);
 }
 
+// ---- rectangular light sources: rays_from_lights (src/ray_generators.jl:64-113) -------------
+__global__ __launch_bounds__(256) void k_raygen_lights(const LightSet A, splat_adray_t* __restrict__ rays, int64_t R) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= R) return;
+  rays[q] = light_ray(A, q);
+}
+
 // ---- backward shading (continuum_shade!, src/ray_imagers.jl:440-503) ----------------------------
 // one thread per ray (rays in pixel order on an h-row grid): final_evolution in place, then its
 // n_delta^2 samples into the (h * n_delta)-row interleaved RGB image
@@ -251,6 +258,10 @@ void launch_reorder(const splat_adray_t* src, const int32_t* hsrc, int64_t R, sp
 void launch_raygen(const splat_pinhole_t& cam, splat_adray_t* rays, int64_t R, hipStream_t s) {
   if (R <= 0) return;
   hipLaunchKernelGGL(k_raygen, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, cam, rays, R);
+}
+void launch_raygen_lights(const LightSet& A, splat_adray_t* rays, int64_t R, hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(k_raygen_lights, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, A, rays, R);
 }
 static unsigned stream_grid(int64_t n) {
   int64_t g = (n + 255) / 256;

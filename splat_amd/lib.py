@@ -57,10 +57,13 @@ SIGNATURES = {
     "splat_partition_active": (i32, [P, P, i64, P, P, P, sz, P]),
     "splat_light_map": (i32, [P, i64, P, P, P, i32, P, P, P]),
     "splat_raygen_pinhole": (i32, [P, P, i64, P]),
+    "splat_lights_ray_count": (i64, [P, i32, i32]),
+    "splat_raygen_lights": (i32, [P, i32, P, P, i64, P]),
     "splat_fill_f32": (i32, [P, i64, f32, P]),
     "splat_run_ws_bytes": (sz, [i64]),
     "splat_run_evolution": (i32, [P, i64, P, P, i32, P, P, P, sz, P, P]),
     "splat_trace_frame": (i32, [P, P, i64, P, P, i32, P, P, P, P, P, sz, P]),
+    "splat_trace_lights": (i32, [P, i32, P, P, i64, P, P, i32, P, P, P, P, P, sz, P]),
     "splat_param_sizes": (i32, [P]),
     "splat_reorder_ws_bytes": (sz, [i64]),
     "splat_reorder_by_dest": (i32, [P, i64, P, P, sz, P]),

@@ -69,3 +69,20 @@ def small_c2(subdiv: int = 2) -> scene.Scene:
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     nb = float(np.linalg.norm(b.astype(np.float64)))
     return float(np.linalg.norm(a.astype(np.float64) - b.astype(np.float64))) / max(nb, 1e-300)
+
+
+def canonical_nans(rays: np.ndarray) -> np.ndarray:
+    """A copy of ADRay records with every float NaN replaced by one quiet NaN. The reference's
+    arithmetic can make NaN tangents (its d/dy taken at r.x, src/tracer.jl:138-141); x86 and the
+    GPU make default NaNs of opposite sign, and only the sign and payload may differ."""
+    out = np.array(rays, copy=True)
+    for name in out.dtype.names:
+        f = out[name]
+        if f.dtype.kind == "f":
+            f[np.isnan(f)] = np.float32(np.nan)
+    return out
+
+
+def same_rays(a: np.ndarray, b: np.ndarray) -> bool:
+    """Bit-exact equality of ray records, any NaN equal to any NaN (canonical_nans)."""
+    return a.shape == b.shape and canonical_nans(a).tobytes() == canonical_nans(b).tobytes()

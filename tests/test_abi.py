@@ -77,6 +77,27 @@ def test_argument_errors_without_a_gpu():
                                None) == -4
 
 
+def test_light_ray_counts_without_a_gpu():
+    """rays_from_lights sizing (src/ray_generators.jl:64-113): (res1/up) x (res2/up) per light; with
+    several lights each must fill its res1*res2/up^2 slice; bad counts are rejected."""
+    lib = L.lib()
+    A = abi.rect_light((0, 0, 8), (0, 0, -1), (1, 0, 0), (0, 1, 0), 64, 48)
+    B = abi.rect_light((0, 0, 8), (0, 0, -1), (1, 0, 0), (0, 1, 0), 37, 22)
+    two = (abi.RectLight * 2)(A, B)
+    assert lib.splat_lights_ray_count(two, 1, 1) == 64 * 48
+    assert lib.splat_lights_ray_count(two, 1, 4) == 16 * 12
+    assert lib.splat_lights_ray_count(two, 2, 1) == 64 * 48 + 37 * 22
+    assert lib.splat_lights_ray_count(C.byref(two, C.sizeof(abi.RectLight)), 1, 2) == 18 * 11  # one light: no slice
+    assert lib.splat_lights_ray_count(two, 2, 2) == -1  # 37 * 22 / 4 = 203 != 18 * 11
+    assert lib.splat_lights_ray_count(two, 1, 0) == -1
+    assert lib.splat_lights_ray_count(None, 0, 1) == 0
+    assert lib.splat_lights_ray_count(two, abi.MAX_LIGHTS + 1, 1) == -1
+    p = abi.light_rays(1, lambda_mode=abi.LAMBDA_STRATIFIED)
+    assert lib.splat_raygen_lights(two, 1, L.ref(p), C.c_void_p(8), 64 * 48, None) == -1  # no strata for lights
+    p = abi.light_rays(1)
+    assert lib.splat_raygen_lights(two, 1, L.ref(p), C.c_void_p(8), 100, None) == -1  # R must equal the count
+
+
 def test_retina_curves_normalised():
     for xs, ys in abi.retina_curves():
         assert xs[0] == 399 and xs[-1] == 700 and np.all(np.diff(xs) > 0)

@@ -46,6 +46,50 @@ def test_raygen_bitexact(order, lmode, spp, jitter):
     assert got.tobytes() == want.tobytes()
 
 
+def _lights():
+    return [
+        abi.rect_light((0.0, 0.0, 8.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), 64, 48),
+        abi.rect_light((-50.0, 50.0, 0.0), (1.0, -1.0, 0.0), (14.0, 14.0, 0.0), (0.0, 0.0, 20.0), 32, 32),
+        abi.rect_light((1.5, -2.0, 3.0), (0.3, 0.4, -0.866), (0.7, 0.1, -0.2), (0.05, 0.9, 0.3), 37, 22),
+    ]
+
+
+@pytest.mark.parametrize("which,up,lmode,jitter", [
+    ((0,), 1, abi.LAMBDA_FIXED, 1.0),
+    ((0, 1), 2, abi.LAMBDA_FIXED, 1.0),
+    ((2,), 1, abi.LAMBDA_UNIFORM, 0.5),
+    ((2,), 2, abi.LAMBDA_FIXED, 1.0),  # one light: (37 / 2) x (22 / 2) cells, no slice check
+    ((0, 1, 2), 1, abi.LAMBDA_UNIFORM, 1.0),
+])
+def test_rays_from_lights_bitexact(which, up, lmode, jitter):
+    lights = [_lights()[i] for i in which]
+    p = abi.light_rays(up, lambda_mode=lmode, lambda_min=420.0, lambda_max=680.0, jitter=jitter, seed=9)
+    want = O.raygen_lights(lights, p)
+    got = api.from_device(api.rays_from_lights(lights, p), abi.ADRAY)
+    assert got.shape == want.shape
+    assert got.tobytes() == want.tobytes()
+
+
+def test_rays_from_lights_edges():
+    import torch
+
+    # no lights: nothing to generate
+    assert api.rays_from_lights([], abi.light_rays()).shape[0] == 0
+    # one row: the integer half height is 0 and the reference divides by it; the GPU gives the
+    # same infinities and NaNs as the oracle
+    L = abi.rect_light((0.0, 0.0, 1.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), 1, 8)
+    p = abi.light_rays(1, seed=4)
+    want = O.raygen_lights([L], p)
+    got = api.from_device(api.rays_from_lights([L], p), abi.ADRAY)
+    for f in ("pos", "dpdx", "dpdy", "x", "y"):
+        np.testing.assert_array_equal(got[f], want[f])  # NaN == NaN here
+    # two lights whose grids do not divide by the upscale: rejected, nothing written
+    bad = [_lights()[0], _lights()[2]]
+    with pytest.raises(api.L.SplatError):
+        api.rays_from_lights(bad, abi.light_rays(2))
+    torch.cuda.synchronize()
+
+
 # ---- hitters ------------------------------------------------------------------------------
 
 def _bf_hits(rays_np, ntris_np, waves=0):
@@ -317,6 +361,35 @@ def test_run_evolution_bitexact(hitter_kind, depth, force_rand):
     assert int(tr.d_count.item()) == want_live
     assert (tr.hit_idx.cpu().numpy() == want_h).all()
     assert api.from_device(rays_d, abi.ADRAY).tobytes() == want_r.tobytes()
+
+
+@pytest.mark.parametrize("up,hitter_kind", [(1, "bf"), (2, "bv")])
+def test_trace_lights_vs_oracle(up, hitter_kind):
+    """trace!'s forward pass from two RectLights (rays_from_lights -> run_evolution! ->
+    continuum_light_map2!) in one call: rays and hits bit-exact, map at 1e-4 rel L2."""
+    sc = scene.scene_c1()
+    lights = [abi.rect_light((0.0, 0.0, 6.0), (0.0, 0.0, -1.0), (1.5, 0.0, 0.0), (0.0, 1.5, 0.0), 64 * up, 48 * up),
+              abi.rect_light((4.0, 0.5, 2.0), (-0.8, 0.0, -0.6), (0.0, 1.2, 0.0), (0.6, 0.0, -0.8), 32 * up, 32 * up)]
+    lp = abi.light_rays(up, lambda_mode=abi.LAMBDA_UNIFORM, lambda_min=400.0, lambda_max=700.0, seed=13)
+    p = abi.run_params(3, sc.first_diffuse, math.nan, seed=17)
+    lm = abi.lightmap_params(96, 96, n_delta=up)
+    tris_d = api.to_device(sc.ftris)
+    if hitter_kind == "bf":
+        hitter = api.HIPBruteForceHitter(tris_d)
+    else:
+        bvs, mem, n_top = bv.tree_partition(sc.ftris, 3)
+        hitter = api.HIPBVHitter(tris_d, bvs, mem, n_top=n_top)
+    fr = api.LightFrame(lights, lp, hitter, tris_d, p, lm)
+    fr.trace()
+    _sync()
+    want_r, want_h, _ = O.run_evolution(O.raygen_lights(lights, lp), sc.ftris, p)
+    want_tex, want_r2 = O.light_map(want_r, want_h, sc.ftris, lm)
+    assert fr.R == 64 * 48 + 32 * 32
+    assert (fr.hit_idx.cpu().numpy() == want_h).all()
+    # bit-exact but for the sign of NaNs: the light rays' x, y are cell coordinates, so the
+    # reference's d/dy at r.x makes NaN tangents on some rays, on both sides
+    assert H.same_rays(api.from_device(fr.rays, abi.ADRAY), want_r2)
+    assert want_tex.sum() > 0 and H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
 
 
 def test_trace_frame_vs_oracle():

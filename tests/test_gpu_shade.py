@@ -72,32 +72,44 @@ def test_shade_vs_oracle(nd, with_sky):
     assert (got > 0).any() and st.size == got.shape[0]
 
 
-def test_trace_iteration_vs_oracle():
-    """One trace! iteration: forward spectral light map, backward camera image."""
+@pytest.mark.parametrize("source", ["pinhole", "rect_lights"])
+def test_trace_iteration_vs_oracle(source):
+    """One trace! iteration: forward spectral light map (from the point-light pinhole, or from
+    RectLights as trace! does), backward camera image."""
     import torch
 
     sc = scene.scene_c1()
     tris = api.to_device(sc.ftris)
     spectrum = np.arange(400.0, 701.0, 25.0, dtype=np.float32)
-    light = scene.camera_c1(seed=3, height=96, width=96, lambda_mode=abi.LAMBDA_UNIFORM)
     cam = scene.view_camera(40, 48, pos=(2.5, -3.0, 2.5), look_at=(0.0, 0.0, -0.5), seed=4)
     sky = scene.skydome(64, 32)
     bvs, mem, n_top = bv.tree_partition(sc.ftris, 3)
     hitter = api.HIPBVHitter(tris, bvs, mem, n_top=n_top)
-    rgb, tex = api.trace_(forward_hitter=hitter, backward_hitter=api.HIPBruteForceHitter(tris), cam=cam, light=light,
-                          tris=tris, depth=3, first_diffuse=sc.first_diffuse, spectrum=spectrum, tex_width=128,
-                          tex_height=128, skydome=torch.from_numpy(sky).cuda(), seed=21)
+    if source == "pinhole":
+        light = scene.camera_c1(seed=3, height=96, width=96, lambda_mode=abi.LAMBDA_UNIFORM)
+        src = dict(light=light)
+        up = 1
+    else:
+        lights = [abi.rect_light((0.0, 0.0, 6.0), (0.0, 0.0, -1.0), (1.5, 0.0, 0.0), (0.0, 1.5, 0.0), 192, 192)]
+        up = 2
+        lp = abi.light_rays(up, lambda_mode=abi.LAMBDA_UNIFORM, lambda_min=400.0, lambda_max=700.0, seed=3)
+        src = dict(lights=lights, light_params=lp)
+    rgb, tex = api.trace_(forward_hitter=hitter, backward_hitter=api.HIPBruteForceHitter(tris), cam=cam, tris=tris,
+                          depth=3, first_diffuse=sc.first_diffuse, spectrum=spectrum, tex_width=128, tex_height=128,
+                          skydome=torch.from_numpy(sky).cuda(), seed=21, **src)
     _sync()
     # the oracle's composition of the same steps
-    lm = abi.lightmap_params(128, 128, spectrum=spectrum, mode="spectral")
+    lm = abi.lightmap_params(128, 128, n_delta=up, spectrum=spectrum, mode="spectral")
     p_f = abi.run_params(3, sc.first_diffuse, math.nan, seed=21)
-    r_f, h_f, _ = O.run_evolution(O.raygen(light), sc.ftris, p_f)
+    fwd = O.raygen(light) if source == "pinhole" else O.raygen_lights(lights, lp)
+    r_f, h_f, _ = O.run_evolution(fwd, sc.ftris, p_f)
     want_tex, _ = O.light_map(r_f, h_f, sc.ftris, lm)
     assert H.rel_l2(tex.cpu().numpy(), want_tex) < 1e-4
     p_b = abi.run_params(3, sc.first_diffuse, math.nan, seed=22)
     r_b, h_b, _ = O.run_evolution(O.raygen(cam), sc.ftris, p_b)
     r_b, h_b = O.reorder(r_b, h_b)
-    want_rgb, _ = O.shade(r_b, h_b, sc.ftris, lm, want_tex, sky, cam.height)
+    lm_b = abi.lightmap_params(128, 128, spectrum=spectrum, mode="spectral")  # backward upscale 1
+    want_rgb, _ = O.shade(r_b, h_b, sc.ftris, lm_b, want_tex, sky, cam.height)
     got = rgb.cpu().numpy()
     assert want_rgb.max() > 0.05
     assert np.abs(got - want_rgb).max() < 1e-4  # the light map's atomic order, through bilinear reads
