@@ -247,6 +247,36 @@ def served(n_requests: int = 120, rate: float = 40.0):
     return out
 
 
+def bv_build(reps: int = 10):
+    """The tree hitter's BV structure built on the device (splat_hitset_build_bv, one stream
+    synchronisation per build) vs the host build (bv.tree_partition + splat_hitset_prepare_bv),
+    on the 10k- and 178k-triangle scenes: what an animated scene pays per frame."""
+    import numpy as np
+    import torch
+
+    from splat_amd import api, bv, scene
+
+    out = {}
+    for name, sc in (("10k_tris", scene.scene_c2(seed=2)), ("178k_tris", scene.scene_c3())):
+        tris = api.to_device(sc.ftris)
+        h = api.HIPBVHitter.build_on_device(tris)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            h.rebuild(tris)
+            ts.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        bvs, mem, n_top = bv.tree_partition(sc.ftris, h.depth)
+        api.HIPBVHitter(tris, bvs, mem, n_top=n_top)
+        torch.cuda.synchronize()
+        host = time.perf_counter() - t
+        out[name] = {"triangles": sc.n_tris, "leaves": 2 ** h.depth, "device_ms": round(float(np.median(ts)) * 1e3, 3),
+                     "host_ms": round(host * 1e3, 1)}
+        del h, tris
+    return out
+
+
 def render_frame(frames: int = 20):
     """One trace! iteration per step at 1 MP on the 10,240-triangle scene: forward light map from
     1024^2 light rays (uniform wavelengths) into 13 spectral planes of 1024^2 (400-700 nm every
@@ -323,6 +353,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-bv-build", action="store_true")
     ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a 256 MiB device-to-device copy, repeated: calibrates HBM byte counters")
@@ -441,6 +472,7 @@ def main() -> int:
     serve_1mp = None if args.no_served else served_1mp()
     torch.cuda.empty_cache()
     render = None if args.no_served else render_frame()
+    bvb = None if args.no_bv_build else bv_build()
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, params, lm)
     small = None if args.no_cpu_baseline else small_frame()
     out = {
@@ -498,6 +530,7 @@ def main() -> int:
         "served": serve_out,
         "served_1mp": serve_1mp,
         "render": render,
+        "bv_build": bvb,
         "cpu_baseline": cpu,
         "small_frame": small,
     }

@@ -258,6 +258,20 @@ int32_t splat_hitset_prepare_bf_ftri(const splat_ftri_t* tris, int32_t n_tris_le
  * first (meant for a few large triangles -- a ground plane -- that would otherwise inflate every
  * box above them). Any memberships that cover the triangles give the brute-force result. */
 size_t splat_hitset_bv_bytes(int32_t n_bv, int64_t total_members);
+
+/* The BV structure built on the device from the triangles alone (bv_partition + pack_bv_tris +
+ * the DPBVHitter constructor, src/bv.jl:68-136, src/alg_types.jl:81-90, for the tree hitter):
+ * 1 + 2^depth BVs -- BV 0 holds the large triangles (bounding-box diagonal > big_factor x the
+ * median, at most max_big, largest first), tested by every ray; BVs 1.. are the object-median
+ * leaves of the rest (split at the median centroid of each node's widest centroid axis, ties
+ * to the lower index), some possibly empty -- with the binary tree over the leaves. The same
+ * memberships as splat_amd.bv.tree_partition on the host. One stream synchronisation at the
+ * end. `ws` (splat_hitset_build_bv_bytes) holds the hitset's arrays first; the rest of it is
+ * build scratch, free again on return. */
+size_t splat_hitset_build_bv_bytes(int32_t n_tris_len, int32_t depth);
+int32_t splat_hitset_build_bv(const splat_ftri_t* tris, int32_t n_tris_len, int32_t depth,
+                              float big_factor, int32_t max_big, void* ws, size_t ws_bytes,
+                              splat_hitset_t* out, void* stream);
 int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t n_tris_len, int32_t n_bv,
                                 const int64_t* bv_tri_count, const int64_t* bv_tris,
                                 int64_t bv_stride, int32_t group_size, int32_t n_top, void* ws,

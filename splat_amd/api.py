@@ -105,6 +105,36 @@ class HIPBVHitter:
                                             self.mat.ctypes.data_as(C.c_void_p), stride, group_size, n_top, L.ptr(self.ws),
                                             self.ws.numel(), L.ref(self.hs), L.stream_handle()), "prepare_bv")
 
+    @classmethod
+    def build_on_device(cls, tris, depth: int | None = None, big_factor: float = 16.0, max_big: int = 32):
+        """The tree hitter's BVs built on the device from the triangles alone
+        (splat_hitset_build_bv): the memberships bv.tree_partition(depth, big_factor, max_big)
+        makes on the host, with the large-triangle list always present (possibly empty) and
+        empty leaves kept. depth defaults to bv.tree_depth of all the triangles."""
+        L.require_gpu()
+        _check_rows(tris, abi.FTRI.itemsize, "tris")
+        self = cls.__new__(cls)
+        self.bvs = self.counts = self.mat = self.stride = None
+        N = _nrows(tris)
+        self.depth = int(depth) if depth is not None else bvmod.tree_depth(N - 1)
+        self.big_factor, self.max_big = float(big_factor), int(max_big)
+        nbytes = L.lib().splat_hitset_build_bv_bytes(N, self.depth)
+        if nbytes == 0:
+            raise ValueError(f"device BV build: unsupported size (N={N}, depth={self.depth})")
+        self.ws = _ws(nbytes, tris.device)
+        self.hs = abi.HitSet()
+        self.rebuild(tris)
+        return self
+
+    def rebuild(self, tris) -> None:
+        """Rebuilds a device-built hit set in place for moved triangles (same count): an animated
+        scene's per-frame bv_partition (src/scenes/splash_scene.jl:41)."""
+        if getattr(self, "depth", None) is None:
+            raise ValueError("rebuild needs a hitter made by build_on_device")
+        L.check(L.lib().splat_hitset_build_bv(L.ptr(tris), _nrows(tris), self.depth, self.big_factor, self.max_big,
+                                              L.ptr(self.ws), self.ws.numel(), L.ref(self.hs), L.stream_handle()),
+                "build_bv")
+
 
 class HIPTracer:
     """ExperimentalTracer(A, rays, N) (src/alg_types.jl:101-115): owns hit_idx (Int32, 1-based),

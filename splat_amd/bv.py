@@ -54,23 +54,30 @@ def tree_depth(n_tris: int, per_leaf: float = 8.0) -> int:
     return max(1, min(20, int(round(math.log2(max(n_tris, 2) / per_leaf)))))
 
 
+def centroids(ftris: np.ndarray) -> np.ndarray:
+    """((a + b) + c) / 3 per triangle in float64, -0 folded into +0 (the device build's keys)."""
+    v = ftris["v"][:, 1:4, :].astype(np.float64)
+    return ((v[:, 0] + v[:, 1]) + v[:, 2]) / 3.0 + 0.0
+
+
 def object_partition(ftris: np.ndarray, depth: int = 8):
     """Object-median build: each triangle (except the sentinel) goes to exactly one leaf by its
     centroid, split at the median along the centroids' widest axis, 2**depth leaves in recursion
-    order (siblings adjacent). Boxes may overlap but nothing is duplicated, so a large triangle
-    (a ground quad) enlarges only its own leaf instead of every leaf it straddles, which is what
-    makes the reference's straddler-duplicating split (bv_partition) cull poorly. Any covering
-    membership gives the same hit results; this one is the fast build for the workloads here.
+    order (siblings adjacent). A node's triangles are ordered by (centroid on that axis, index)
+    and the first half goes left, so equal centroids split by index. Boxes may overlap but
+    nothing is duplicated, so a large triangle (a ground quad) enlarges only its own leaf instead
+    of every leaf it straddles, which is what makes the reference's straddler-duplicating split
+    (bv_partition) cull poorly. Any covering membership gives the same hit results; this one is
+    the fast build for the workloads here, and the one splat_hitset_build_bv makes on the device.
     Returns (bvs, memberships) like bv_partition."""
-    verts = ftris["v"][:, 1:4, :].astype(np.float64)
-    cent = verts.mean(axis=1)
+    cent = centroids(ftris)
 
     def rec(idx: np.ndarray, d: int) -> list[np.ndarray]:
         if d == 0 or idx.size <= 1:
             return [idx]
         c = cent[idx]
         ax = int(np.argmax(c.max(axis=0) - c.min(axis=0)))
-        o = idx[np.argsort(c[:, ax], kind="stable")]
+        o = idx[np.lexsort((idx, c[:, ax]))]
         h = o.size // 2
         return rec(o[:h], d - 1) + rec(o[h:], d - 1)
 
@@ -93,11 +100,12 @@ def tree_partition(ftris: np.ndarray, depth: int | None = None, big_factor: floa
     inflate its leaf's box and every box above it, so every ray would enter them. Returns
     (bvs, memberships, n_top) with n_top = 1 when such triangles exist, else 0."""
     v = ftris["v"][1:, 1:4, :].astype(np.float64)
-    diag = np.linalg.norm(v.max(axis=1) - v.min(axis=1), axis=1)
+    e = v.max(axis=1) - v.min(axis=1)
+    diag = np.sqrt((e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2])
     med = float(np.median(diag)) if diag.size else 0.0
     big = np.nonzero(diag > big_factor * med)[0] if med > 0 else np.zeros(0, dtype=np.int64)
-    if big.size > max_big:
-        big = big[np.argsort(diag[big])[::-1][:max_big]]
+    if big.size > max_big:  # the largest first, equal diagonals by index
+        big = big[np.lexsort((big, -diag[big]))[:max_big]]
     big = np.sort(big) + 2  # 1-based triangle indices (row 0 is the sentinel)
     rest = np.setdiff1d(np.arange(2, ftris.shape[0] + 1, dtype=np.int64), big)
     if depth is None:

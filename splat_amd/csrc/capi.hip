@@ -383,6 +383,55 @@ int32_t splat_hitset_prepare_bv(const splat_ftri_t* tris, int32_t N, int32_t n_b
   return SPLAT_OK;
 }
 
+size_t splat_hitset_build_bv_bytes(int32_t N, int32_t depth) {
+  if (N < 2 || N > (1 << 26) || depth < 1 || depth > 20) return 0;
+  const int32_t n_bv = 1 + (1 << depth);
+  return splat_hitset_bv_bytes(n_bv, N - 1) + bv_build_scratch_bytes(N - 1, depth);
+}
+
+int32_t splat_hitset_build_bv(const splat_ftri_t* tris, int32_t N, int32_t depth, float big_factor, int32_t max_big,
+                              void* ws, size_t ws_bytes, splat_hitset_t* out, void* stream) {
+  if (!tris || !ws || !out || max_big < 0 || !(big_factor > 0.f)) return SPLAT_ERR_INVALID_ARG;
+  const size_t need = splat_hitset_build_bv_bytes(N, depth);
+  if (need == 0) return SPLAT_ERR_UNSUPPORTED;
+  if (ws_bytes < need) return SPLAT_ERR_WORKSPACE;
+  const int32_t n = N - 1, n_bv = 1 + (1 << depth), m = n_bv - 1;
+  // the hitset's arrays, laid out as splat_hitset_prepare_bv lays them; the build's scratch after
+  Carver c(ws);
+  TriRec* recs = c.take<TriRec>((size_t)n);
+  float4* boxes = c.take<float4>((size_t)n_bv * 2);
+  float4* groups = c.take<float4>(top_boxes(n_bv) * 2);
+  int2* d_ranges = c.take<int2>((size_t)n_bv);
+  int32_t* d_members = c.take<int32_t>((size_t)n);
+  void* scratch = static_cast<char*>(ws) + splat_hitset_bv_bytes(n_bv, n);
+  hipStream_t s = S(stream);
+  int32_t summary[2] = {0, 0};
+  if (int32_t e = bv_build_members(tris, n, depth, (double)big_factor, max_big, scratch, d_members, d_ranges, summary, s))
+    return e;
+  const int32_t levels = tree_levels_for(m);
+  launch_prep_members(tris, d_members, n, recs, s);
+  launch_bv_boxes(tris, d_members, d_ranges, n_bv, boxes, s);
+  launch_tree_build(boxes + 2, d_ranges + 1, m, levels, groups, s);
+  if (int32_t e = launch_status()) return e;
+  if (hipStreamSynchronize(s) != hipSuccess) return SPLAT_ERR_HIP;  // the summary is on the host now
+  const int32_t n_big = summary[0], rest = summary[1];
+  std::memset(out, 0, sizeof(*out));
+  out->kind = SPLAT_HITTER_BV;
+  out->n_tris = N;
+  out->n_bv = n_bv;
+  const int32_t leaf_max = (rest + m - 1) / m;
+  out->max_members = n_big > leaf_max ? n_big : leaf_max;
+  out->total_members = n;
+  out->d_recs = recs;
+  out->d_boxes = boxes;
+  out->d_ranges = d_ranges;
+  out->d_groups = groups;
+  out->tree_levels = levels + 1;
+  out->top_bvs = 1;
+  out->top_recs = n_big;
+  return SPLAT_OK;
+}
+
 size_t splat_next_hit_ws_bytes(int64_t R) { return R >= 0 ? carve_hit(nullptr, R).bytes : 0; }
 
 int32_t splat_next_hit(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs,

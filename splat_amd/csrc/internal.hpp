@@ -26,6 +26,12 @@ void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, f
 // centres most, bit 2 = the right child lies lower on it, bit 3 = empty subtree.
 constexpr int32_t TREE_AXIS = 3, TREE_SWAP = 4, TREE_EMPTY = 8;
 constexpr int32_t MAX_TREE_LEVELS = 24;
+// the device BV build (bvbuild.hip): member list and ranges of 1 + 2^depth BVs (the large
+// triangles, then the object-median leaves) of triangles 2..n+1; h_summary <- {n_large, n - n_large}
+// once the stream reaches it
+size_t bv_build_scratch_bytes(int32_t n, int32_t depth);
+int32_t bv_build_members(const splat_ftri_t* tris, int32_t n, int32_t depth, double big_factor, int32_t max_big,
+                         void* scratch, int32_t* members, int2* ranges, int32_t* h_summary, hipStream_t s);
 void launch_tree_build(const float4* boxes, const int2* ranges, int32_t n_bv, int32_t levels, float4* nodes,
                        hipStream_t s);
 

@@ -49,6 +49,8 @@ SIGNATURES = {
     "splat_hitset_prepare_bf_ftri": (i32, [P, i32, P, sz, P, P]),
     "splat_hitset_bv_bytes": (sz, [i32, i64]),
     "splat_hitset_prepare_bv": (i32, [P, i32, i32, P, P, i64, i32, i32, P, sz, P, P]),
+    "splat_hitset_build_bv_bytes": (sz, [i32, i32]),
+    "splat_hitset_build_bv": (i32, [P, i32, i32, f32, i32, P, sz, P, P]),
     "splat_next_hit_ws_bytes": (sz, [i64]),
     "splat_next_hit": (i32, [P, i64, P, P, P, P, sz, P]),
     "splat_next_hit_bf": (i32, [P, i64, P, i32, P, P, sz, P]),

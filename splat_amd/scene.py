@@ -147,6 +147,20 @@ def cube_mesh(centers: np.ndarray, edge: float) -> np.ndarray:
     return mesh_to_ftri(flat_pos, flat_faces, normals_v, uv)
 
 
+def rotate_ftris(ftris: np.ndarray, axis, angle: float) -> np.ndarray:
+    """rotate(t::FTri, R) (src/geo.jl:139-150) with R the rotation by `angle` about `axis`
+    (Rodrigues): the face normal, vertices and vertex normals turn, the uvs stay."""
+    k = np.asarray(axis, dtype=np.float64)
+    k = k / np.linalg.norm(k)
+    K = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+    R = np.eye(3) + math.sin(angle) * K + (1.0 - math.cos(angle)) * (K @ K)
+    out = np.array(ftris, copy=True)
+    v = out["v"].astype(np.float64)
+    v[:, :7, :] = v[:, :7, :] @ R.T
+    out["v"] = v.astype(np.float32)
+    return out
+
+
 def assemble(glass: list[np.ndarray], diffuse: list[np.ndarray], name: str, **meta) -> Scene:
     """[degenerate] + glass + diffuse, first_diffuse = 1 + 1 + n_glass (cf. src/scenes/sphere.jl:36-37)."""
     zero = np.zeros(1, dtype=abi.FTRI)

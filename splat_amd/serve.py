@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import abi, api, bv, scene
+from . import abi, api, scene
 
 
 @dataclass
@@ -27,9 +27,10 @@ class ResidentScene:
 
     @staticmethod
     def build(sc: scene.Scene, depth: int | None = None, device: str = "cuda") -> "ResidentScene":
+        """Uploads the triangles and builds the tree hitter's BVs on the device (0.4-0.8 ms for
+        10k-178k triangles; the host tree_partition takes 30-440 ms)."""
         tris = api.to_device(sc.ftris, device)
-        bvs, mem, n_top = bv.tree_partition(sc.ftris, depth)
-        return ResidentScene(sc, tris, api.HIPBVHitter(tris, bvs, mem, group_size=0, n_top=n_top), sc.name)
+        return ResidentScene(sc, tris, api.HIPBVHitter.build_on_device(tris, depth), sc.name)
 
 
 @dataclass
