@@ -127,7 +127,9 @@ typedef struct splat_hitset {
   int32_t n_groups;      /* ceil(n_bv / group_size)                                        */
   int32_t tune_waves;    /* BF: triangle-split waves per workgroup (0 = autotuned default); */
                          /* BV tree: 0 = waves walk together, 1 = per wave, together when */
-                         /* its rays share a direction octant, else per lane; 2 = per lane */
+                         /* its rays share a direction octant, else per lane; 2 = per lane; */
+                         /* + 4: walk the tree also when few rays are live (by default a   */
+                         /* launch with <= 4096 live rays scans the leaf boxes flat)         */
   int32_t tree_levels;   /* 1 + levels of the tree over the BVs; 0 = no tree (BV only)    */
   int32_t top_bvs;       /* leading BVs whose members every ray tests directly (BV only) */
   int32_t top_recs;      /* their records: d_recs[0, top_recs)                            */

@@ -29,7 +29,7 @@ namespace spl {
 
 constexpr int MAX_SLOTS = 1024;
 constexpr int MAXD = SPLAT_ENGINE_MAX_DEPTH;
-enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, N_KINDS };
+enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, K_WAVE, N_KINDS };  // K_HIT: 64-ray groups; K_WAVE: hitter waves
 constexpr int HIT_ITEM = 64;     // rays per hitter work item (one wave)
 constexpr int CLEAR_ITEM = 4096;  // light-map floats per clear item
 
@@ -58,7 +58,7 @@ struct EngineDev {
   int32_t* stage;      // -1 = empty
   int64_t* counters;   // [slot][MAXD + 2]: live rays after bounce k (k = 0: R)
   int32_t* pre;        // [kind][MAX_SLOTS + 1] exclusive prefixes of work items
-  int32_t* split;      // splits per 64-ray hitter item in this step
+  int32_t* hit_split;  // [slot] splits per 64-ray group this step; negative: flat-scan tail
   int32_t n_slots;
 };
 
@@ -80,7 +80,9 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
   const int t = threadIdx.x;
   const bool in = t < E.n_slots;
   const int32_t s = in ? E.stage[t] : -1;
-  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0};
+  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0, 0};
+  bool tail = false;
+  int32_t leaves = 1;
   if (s >= 0) {
     const SlotDev& D = E.slots[t];
     int64_t* C = E.counters + (int64_t)t * (MAXD + 2);
@@ -94,7 +96,28 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
     cnt[K_HIT] = (int32_t)cdiv(A, HIT_ITEM);
     if (s < D.depth) cnt[K_TILE] = (int32_t)cdiv(A, TILE);
     if (s == D.depth) cnt[K_FINAL] = (int32_t)cdiv(A, TILE);
+    tail = bv_mode(D.scene) == BV_TREE && cnt[K_HIT] <= TAIL_GROUPS && !D.scene.no_tail;
+    leaves = D.scene.n_bv;
   }
+  // the hitter's splits: one factor for the tree walks of all frames, a flat-scan tail split for
+  // frames with few live rays (both from the totals, so the items fit the host's grid)
+  auto block_sum = [&](int32_t v) {
+    buf[t] = v;
+    __syncthreads();
+    for (int off = MAX_SLOTS / 2; off > 0; off >>= 1) {
+      if (t < off) buf[t] += buf[t + off];
+      __syncthreads();
+    }
+    const int32_t r = buf[0];
+    __syncthreads();
+    return r;
+  };
+  const int32_t g_walk = block_sum(tail ? 0 : cnt[K_HIT]);
+  const int32_t g_tail = block_sum(tail ? cnt[K_HIT] : 0);
+  const int32_t S_walk = split_factor(g_walk, MAX_SPLIT);
+  const int32_t S_slot = tail ? tail_split_factor(g_tail, leaves) : S_walk;
+  if (in) E.hit_split[t] = tail ? -S_slot : S_slot;
+  cnt[K_WAVE] = cnt[K_HIT] * S_slot;
   for (int k = 0; k < N_KINDS; ++k) {
     buf[t] = cnt[k];
     __syncthreads();
@@ -109,7 +132,6 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
     if (t == 0) P[E.n_slots] = buf[E.n_slots - 1];
     __syncthreads();
   }
-  if (t == 0) *E.split = split_factor(E.pre[K_HIT * (MAX_SLOTS + 1) + E.n_slots], MAX_SPLIT);
 }
 
 __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
@@ -146,22 +168,26 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
 // frames' ray counts) covers the most items the live counts can need; waves past them leave.
 __global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
   const int32_t n = E.n_slots;
-  const int32_t* P = E.pre + K_HIT * (MAX_SLOTS + 1);
-  const int32_t S = *E.split;
-  const int64_t total = (int64_t)P[n] * S;
+  const int32_t* P = E.pre + K_WAVE * (MAX_SLOTS + 1);
+  const int64_t total = P[n];
   HitStats hs;
   const int64_t w = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w < total) {
     const int32_t i = (int32_t)w;
-    const int32_t item = i / S;
-    const int32_t s = item_slot(P, n, item);
+    const int32_t s = item_slot(P, n, i);
+    const int32_t sv = E.hit_split[s];
+    const bool tail = sv < 0;
+    const int32_t S = tail ? -sv : sv;
+    const int32_t local = i - P[s], group = local / S, sp = local - group * S;
     const SlotDev& D = E.slots[s];
     const int32_t st = E.stage[s];
     const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
-    const int64_t base = (int64_t)(item - P[s]) * HIT_ITEM;
+    const int64_t base = (int64_t)group * HIT_ITEM;
     uint64_t* keys = D.keys[st & 1];
-    const int32_t sp = i - item * S;
-    if (st == 0 && D.depth > 0) {  // the first bounce: the pinhole's rays, generated here
+    if (tail) {  // few live rays: flat over the leaves
+      if (st == 0 && D.depth > 0) bv_item<false, BV_FLAT, true>(nullptr, A, base, sp, S, D.scene, keys, hs, &D.cam);
+      else bv_item<false, BV_FLAT>(D.S[st & 1], A, base, sp, S, D.scene, keys, hs);
+    } else if (st == 0 && D.depth > 0) {  // the first bounce: the pinhole's rays, generated here
       switch (bv_mode(D.scene)) {
         case BV_TREE: bv_item<false, BV_TREE, true>(nullptr, A, base, sp, S, D.scene, keys, hs, &D.cam); break;
         case BV_GROUPS: bv_item<false, BV_GROUPS, true>(nullptr, A, base, sp, S, D.scene, keys, hs, &D.cam); break;
@@ -422,7 +448,7 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   e->max_slots = max_slots;
   size_t dev_bytes = al(sizeof(SlotDev) * max_slots) + al(sizeof(splat_lightmap_params_t) * max_slots) +
                      al(sizeof(int32_t) * max_slots) + al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2)) +
-                     al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1)) + al(sizeof(int32_t));
+                     al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1)) + al(sizeof(int32_t) * MAX_SLOTS);
   if (hipMalloc(&e->arena_base, (size_t)arena_bytes) != hipSuccess) { delete e; return SPLAT_ERR_BUDGET; }
   if (hipMalloc(&e->dev_mem, dev_bytes) != hipSuccess) { (void)hipFree(e->arena_base); delete e; return SPLAT_ERR_HIP; }
   if (hipHostMalloc(reinterpret_cast<void**>(&e->h_slots), sizeof(SlotDev) * max_slots) != hipSuccess ||
@@ -438,7 +464,7 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   e->dev.stage = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * max_slots);
   e->dev.counters = reinterpret_cast<int64_t*>(p); p += al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2));
   e->dev.pre = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1));
-  e->dev.split = reinterpret_cast<int32_t*>(p);
+  e->dev.hit_split = reinterpret_cast<int32_t*>(p);
   e->dev.n_slots = max_slots;
   for (int i = 0; i < max_slots; ++i) e->h_stage0[i] = -1;
   if (hipMemcpy(e->dev.stage, e->h_stage0, sizeof(int32_t) * max_slots, hipMemcpyHostToDevice) != hipSuccess) {

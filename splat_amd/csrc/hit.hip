@@ -160,13 +160,20 @@ __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict_
   const int64_t A = live_count(R, d_count);
   const int64_t groups64 = (A + 63) / 64;
   const int32_t units = MODE == BV_TREE ? (1 << scene.levels) : MODE == BV_GROUPS ? scene.n_groups : scene.n_bv;
-  const int32_t S = split_factor(groups64, units);
+  const bool tail = MODE == BV_TREE && groups64 <= TAIL_GROUPS && !scene.no_tail;  // few rays: flat over the leaves
+  const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units);
   const int64_t i = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (i >= groups64 * S) return;  // whole waves
   HitStats st;
   const int64_t g = i / S;
-  if (from_cam) bv_item<STATS, MODE, true>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st, &cam);
-  else bv_item<STATS, MODE, false>(rays, A, g * 64, (int32_t)(i - g * S), S, scene, keys, st);
+  const int32_t sp = (int32_t)(i - g * S);
+  if (tail) {
+    if (from_cam) bv_item<STATS, BV_FLAT, true>(rays, A, g * 64, sp, S, scene, keys, st, &cam);
+    else bv_item<STATS, BV_FLAT, false>(rays, A, g * 64, sp, S, scene, keys, st);
+  } else {
+    if (from_cam) bv_item<STATS, MODE, true>(rays, A, g * 64, sp, S, scene, keys, st, &cam);
+    else bv_item<STATS, MODE, false>(rays, A, g * 64, sp, S, scene, keys, st);
+  }
   if constexpr (STATS) {
     atomicAdd(stats + 0, st.n_box);
     atomicAdd(stats + 1, st.n_enter);

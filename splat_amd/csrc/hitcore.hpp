@@ -212,14 +212,15 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
   float best = INFINITY;
   int32_t bi = 1;
   // another split of the same rays may already hold a nearer hit: prune with it (it is a real
-  // hit of this ray, so the lexicographic minimum is unchanged)
-  auto refresh = [&]() {
-    if (shared_key && valid) {
-      const uint64_t k = __hip_atomic_load(shared_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (k < pack_key(best, bi)) {
-        best = key_t(k);
-        bi = key_index(k);
-      }
+  // hit of this ray, so the lexicographic minimum is unchanged). The load is issued before a
+  // leaf's members are scanned and merged after, so its latency hides behind the scan.
+  auto fetch = [&]() -> uint64_t {
+    return shared_key && valid ? __hip_atomic_load(shared_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
+  };
+  auto merge = [&](uint64_t k) {
+    if (k < pack_key(best, bi)) {
+      best = key_t(k);
+      bi = key_index(k);
     }
   };
   // every lane tests the members of leaf b (some lane entered it)
@@ -272,7 +273,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     const cptr<float4> groups = as_const(S.groups);
     const int32_t n_bv = S.n_bv, gsize = S.gsize, n_groups = S.n_groups;
     for (int32_t g = split; g < n_groups; g += n_split) {
-      refresh();
+      merge(fetch());
       const float4 glo = groups[2 * g], ghi = groups[2 * g + 1];
       const bool enter = valid && slab_enter(glo, ghi, L, ix, iy, iz, best);
       if constexpr (STATS) n_box += valid ? 1ull : 0ull;
@@ -320,8 +321,9 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       bool go = test(nlo, nodes[2 * root + 1]);
       while (go) {  // some lane entered node n
         if (n >= leaf0) {
-          refresh();
+          const uint64_t k = fetch();
           scan_members(n - leaf0, lane_in);
+          merge(k);
         } else {
           const int32_t code = __float_as_int(nlo.w);
           const int32_t near = ((neg >> (code & TREE_AXIS)) & 1) ^ ((code & TREE_SWAP) ? 1 : 0);
@@ -366,10 +368,29 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       }
     }
   } else {
+    // leaves split, split + n_split, ...: eight box loads and tests at a time (independent
+    // loads, no chain), then the members of the leaves some lane entered
     const int32_t n_bv = S.n_bv;
-    for (int32_t b = split; b < n_bv; b += n_split) {
-      if (((b - split) & 15) == 0) refresh();
-      scan_leaf(b);
+    for (int32_t b0 = split; b0 < n_bv; b0 += 8 * n_split) {
+      const uint64_t k = fetch();
+      uint32_t wave_mask = 0, lane_mask = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int32_t b = b0 + j * n_split;
+        if (b < n_bv) {
+          const float4 lo = boxes[2 * b], hi = boxes[2 * b + 1];
+          const bool e = valid && slab_enter(lo, hi, L, ix, iy, iz, best);
+          if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+          if (__builtin_amdgcn_ballot_w64(e) != 0) wave_mask |= 1u << j;
+          lane_mask |= (e ? 1u : 0u) << j;
+        }
+      }
+      while (wave_mask != 0) {
+        const int j = __builtin_ctz(wave_mask);
+        wave_mask &= wave_mask - 1;
+        scan_members(b0 + j * n_split, ((lane_mask >> j) & 1u) != 0);
+      }
+      merge(k);
     }
   }
   if (best_io) *best_io = best;
@@ -385,10 +406,25 @@ __device__ __forceinline__ int32_t split_factor(int64_t groups64, int32_t units)
   while (S < MAX_SPLIT && S < units && groups64 * S < SPLIT_TARGET_ITEMS) S *= 2;
   return S;
 }
+// A launch with at most TAIL_GROUPS live 64-ray groups (the last bounces: a few thousand rays
+// trapped in glass, pointing every way) scans the tree's leaf boxes flat, split over many items:
+// the tree walk of such a wave is one long chain of dependent node loads, the flat scan is
+// independent loads. Split: up to TAIL_MAX_SPLIT items per group, at least 8 leaves each.
+constexpr int64_t TAIL_GROUPS = 64;
+constexpr int32_t TAIL_MAX_SPLIT = 128;
+constexpr int64_t TAIL_ITEMS = 8192;  // at most this many flat-scan items per launch
+// split of `groups64` tail groups (in all) over scenes of `leaves` leaves: S * 8 <= leaves,
+// groups64 * S <= TAIL_ITEMS
+__device__ __forceinline__ int32_t tail_split_factor(int64_t groups64, int32_t leaves) {
+  int32_t S = 1;
+  while (S < TAIL_MAX_SPLIT && S * 16 <= leaves && groups64 * (S * 2) <= TAIL_ITEMS) S *= 2;
+  return S;
+}
 // Work items (waves) the hitter needs for at most `groups64` live groups: the split factor only
-// grows while groups64 * S < SPLIT_TARGET_ITEMS, so it never takes the count past twice that.
+// grows while groups64 * S < SPLIT_TARGET_ITEMS, so it never takes the count past twice that;
+// flat-scan tails add at most TAIL_ITEMS.
 inline __host__ __device__ int64_t max_hit_items(int64_t groups64) {
-  return groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS;
+  return (groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS) + TAIL_ITEMS;
 }
 
 // One (64-ray group, split) work item: rays [base, base + 64) of `rays` (CAM: of the pinhole) with

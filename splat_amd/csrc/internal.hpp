@@ -46,8 +46,9 @@ struct BVScene {
   int32_t n_bv, gsize, n_groups;
   const float4* nodes;
   int32_t levels;
-  int32_t walk;   // tree only: BV_WALK_*
-  int32_t n_top;  // records recs[0, n_top) that every ray tests before the structure
+  int32_t walk;     // tree only: BV_WALK_*
+  int32_t n_top;    // records recs[0, n_top) that every ray tests before the structure
+  int32_t no_tail;  // tree only: walk the tree even for launches with few live rays
 };
 // How a wave walks the tree: together (one path, the union of its rays'; scalar node loads), or
 // each lane its own path; AUTO: together when all its rays point into one octant.
@@ -69,7 +70,9 @@ inline BVScene bv_scene(const splat_hitset_t* hs) {
   b.n_groups = g ? hs->n_groups : 0;
   b.nodes = tree ? static_cast<const float4*>(hs->d_groups) : nullptr;
   b.levels = tree ? hs->tree_levels - 1 : 0;
-  b.walk = tree && hs->tune_waves >= BV_WALK_WAVE && hs->tune_waves <= BV_WALK_LANES ? hs->tune_waves : BV_WALK_WAVE;
+  const int32_t w = hs->tune_waves & 3;
+  b.walk = tree && w <= BV_WALK_LANES ? w : BV_WALK_WAVE;
+  b.no_tail = tree && (hs->tune_waves & 4) ? 1 : 0;
   return b;
 }
 

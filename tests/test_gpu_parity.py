@@ -190,15 +190,20 @@ def test_next_hit_bv_equals_bf(build, depth, group_size):
     assert (got == want).all()
 
 
-@pytest.mark.parametrize("depth,group_size,walk", [(3, 0, 0), (5, 0, 1), (5, 0, 2), (4, 4, 0), (4, -1, 0), (1, 0, 0)])
-def test_next_hit_bv_top_list_equals_bf(depth, group_size, walk):
+@pytest.mark.parametrize("depth,group_size,walk,n_rays", [
+    (3, 0, 0, None), (5, 0, 1, None), (5, 0, 2, None), (4, 4, 0, None), (4, -1, 0, None), (1, 0, 0, None),
+    # at most 64 live 64-ray groups: the tree's leaves are scanned flat, unless walk has bit 4
+    (5, 0, 0, 3000), (8, 0, 0, 4096), (1, 0, 0, 100), (5, 0, 4, 3000), (5, 0, 5, 3000), (5, 0, 6, 3000),
+])
+def test_next_hit_bv_top_list_equals_bf(depth, group_size, walk, n_rays):
     """Large triangles tested directly before the structure (tree_partition), every structure
-    and tree walk: the brute-force result."""
+    and tree walk, and the flat scan of launches with few rays: the brute-force result."""
     sc = H.small_c2(4)
     cam = scene.camera_c2(seed=4, height=64, width=64)
     rays = O.raygen(cam)
     pts = H.sample_surface_points(sc.ftris, 2000, seed=15)
-    rays = np.concatenate([rays, H.rays_towards(pts, seed=16), H.random_rays(1000, seed=17, n_tris=sc.n_tris)])
+    rays = np.concatenate([H.rays_towards(pts, seed=16), H.random_rays(1000, seed=17, n_tris=sc.n_tris), rays])
+    rays = rays[:n_rays] if n_rays else rays
     want, _ = O.next_hit(rays, sc.ntris())
     tris_d = api.to_device(sc.ftris)
     bvs, mem, n_top = bv.tree_partition(sc.ftris, depth)
