@@ -64,6 +64,40 @@ __device__ __forceinline__ bool slab_enter(const float4& lo, const float4& hi, c
   return (tmin <= tmax) && (tmin <= best);
 }
 
+// keep (t, idx) when it is a hit lexicographically before (best, bi); selects, not branches, so
+// the record loads of consecutive members can be issued ahead of their tests
+__device__ __forceinline__ void take_nearer(bool h, float t, int32_t idx, float& best, int32_t& bi) {
+  const bool take = h && (t < best || (t == best && idx < bi));
+  best = take ? t : best;
+  bi = take ? idx : bi;
+}
+
+// Up to 16 consecutive records recs[0, cnt) tested by the whole wave: one coalesced vector load
+// brings them in (lane l holds quarter l % 4 of record l / 4), each record is then broadcast to
+// every lane with readlane. Scalar loads return out of order, so a loop of them waits for each
+// record in turn; this waits once per 16.
+__device__ __forceinline__ float bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ void scan_records16(const TriRec* __restrict__ recs, int32_t cnt, const RayLanes& L,
+                                               float& best, int32_t& bi) {
+  const int lane = threadIdx.x & 63;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if ((lane >> 2) < cnt) v = reinterpret_cast<const float4*>(recs + (lane >> 2))[lane & 3];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (j < cnt) {
+      float4 q[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        q[c] = make_float4(bcast(v.x, 4 * j + c), bcast(v.y, 4 * j + c), bcast(v.z, 4 * j + c), bcast(v.w, 4 * j + c));
+      float t;
+      const bool h = hit_rec(q[0], q[1], q[2], q[3], L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
+      take_nearer(h, t, __float_as_int(q[1].w), best, bi);
+    }
+  }
+}
+
 struct HitStats {
   unsigned long long n_box = 0, n_enter = 0, n_alg = 0, n_exec = 0;
 };
@@ -183,10 +217,7 @@ __device__ __forceinline__ void tree_walk_lanes(const RayLanes& L, bool valid, f
         float t;
         const bool h = hit_rec(q0, q1, q2, q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
         const int32_t idx = __float_as_int(q1.w);
-        if (h && (t < best || (t == best && idx < bi))) {
-          best = t;
-          bi = idx;
-        }
+        take_nearer(h, t, idx, best, bi);
       }
     }
     if (leaf >= 0) next();
@@ -199,7 +230,6 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
                                                const uint64_t* shared_key = nullptr) {
   const cptr<float4> boxes = as_const(S.boxes);
   const cptr<int2> ranges = as_const(S.ranges);
-  const cptr<TriRec> recs = as_const(S.recs);
   unsigned long long& n_box = st.n_box;
   unsigned long long& n_enter = st.n_enter;
   unsigned long long& n_alg = st.n_alg;
@@ -231,17 +261,8 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       n_alg += entered ? (unsigned long long)rg.y : 0ull;
       n_exec += (unsigned long long)rg.y;
     }
-#pragma unroll 4
-    for (int32_t m = 0; m < rg.y; ++m) {
-      const float4 q0 = recs[rg.x + m].q0, q1 = recs[rg.x + m].q1, q2 = recs[rg.x + m].q2, q3 = recs[rg.x + m].q3;
-      float t;
-      bool h = hit_rec(q0, q1, q2, q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
-      int32_t idx = __float_as_int(q1.w);
-      if (h && (t < best || (t == best && idx < bi))) {
-        best = t;
-        bi = idx;
-      }
-    }
+    for (int32_t m = 0; m < rg.y; m += 16)
+      scan_records16(S.recs + rg.x + m, rg.y - m < 16 ? rg.y - m : 16, L, best, bi);
   };
   // the records every ray tests before the structure (large triangles)
   if (S.n_top > 0) {
@@ -250,16 +271,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       n_alg += valid ? (unsigned long long)nt : 0ull;
       n_exec += (unsigned long long)nt;
     }
-    for (int32_t m = 0; m < nt; ++m) {
-      const float4 q0 = recs[m].q0, q1 = recs[m].q1, q2 = recs[m].q2, q3 = recs[m].q3;
-      float t;
-      bool h = hit_rec(q0, q1, q2, q3, L.ox, L.oy, L.oz, L.dx, L.dy, L.dz, L.ig, t);
-      int32_t idx = __float_as_int(q1.w);
-      if (h && (t < best || (t == best && idx < bi))) {
-        best = t;
-        bi = idx;
-      }
-    }
+    for (int32_t m = 0; m < nt; m += 16) scan_records16(S.recs + m, nt - m < 16 ? nt - m : 16, L, best, bi);
   }
   auto scan_leaf = [&](int32_t b) {
     const float4 blo = boxes[2 * b], bhi = boxes[2 * b + 1];
