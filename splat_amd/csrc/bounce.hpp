@@ -188,7 +188,9 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
           float I;
           has = splat_sample(o, F, lm, lam, 0, 0, &sx, &sy, &I);
           if (has)
-            for (int c = 0; c < splat_planes(lm) && c < 3; ++c) sv[c] = splat_value(lm, 0, c, lam, I);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              if (c < splat_planes(lm)) sv[c] = splat_value(lm, 0, c, lam, I);
         } else {
           final_evolve_splat(o, i, F, lm, P.tex);
         }

@@ -158,12 +158,28 @@ __device__ __forceinline__ float bounce_random(float force_rand, const float* __
 }
 
 // ---- final_evolution + atomic splat (src/ray_imagers.jl:4-37, :78-113) ---------------------
+// Piecewise-linear curve c at lam: segment m = the first with lam < x[m + 1] (at most n - 2).
+// The breakpoints are the same for every lane, so the segment is found by one uniform pass
+// over them that advances with selects (no per-lane loop, no per-lane indexing of the
+// parameters); the same segment and the same arithmetic as a scan from the left.
 __device__ __forceinline__ float curve_eval(const splat_lightmap_params_t& lm, int c, float lam) {
-  int n = lm.curve_n[c];
-  int m = 0;
-  while (m < n - 2 && lam >= lm.curve_x[c][m + 1]) ++m;
-  float x0 = lm.curve_x[c][m], x1 = lm.curve_x[c][m + 1];
-  float y0 = lm.curve_y[c][m], y1 = lm.curve_y[c][m + 1];
+  const int n = lm.curve_n[c];
+  float x0 = lm.curve_x[c][0], x1 = lm.curve_x[c][1];
+  float y0 = lm.curve_y[c][0], y1 = lm.curve_y[c][1];
+  bool go = true;
+#pragma unroll 4
+  for (int j = 1; j < n - 1; ++j) {
+    {
+      // values first, then selects (a select of the parameters' addresses would put them in
+      // private memory)
+      const float xa = lm.curve_x[c][j], ya = lm.curve_y[c][j], xb = lm.curve_x[c][j + 1], yb = lm.curve_y[c][j + 1];
+      go = go && lam >= xa;
+      x0 = go ? xa : x0;
+      y0 = go ? ya : y0;
+      x1 = go ? xb : x1;
+      y1 = go ? yb : y1;
+    }
+  }
   float slope = (y1 - y0) / (x1 - x0);
   return FMA(slope, lam - x1, y1);
 }
