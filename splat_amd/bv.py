@@ -60,25 +60,47 @@ def centroids(ftris: np.ndarray) -> np.ndarray:
     return ((v[:, 0] + v[:, 1]) + v[:, 2]) / 3.0 + 0.0
 
 
+SAH_MAX = 4096  # nodes of at most this many triangles choose their split axis by surface area
+
+
+def box_area(lo: np.ndarray, hi: np.ndarray) -> float:
+    d = np.maximum(hi - lo, 0.0)
+    return 2.0 * ((d[0] * d[1] + d[1] * d[2]) + d[2] * d[0])
+
+
 def object_partition(ftris: np.ndarray, depth: int = 8):
     """Object-median build: each triangle (except the sentinel) goes to exactly one leaf by its
-    centroid, split at the median along the centroids' widest axis, 2**depth leaves in recursion
-    order (siblings adjacent). A node's triangles are ordered by (centroid on that axis, index)
-    and the first half goes left, so equal centroids split by index. Boxes may overlap but
-    nothing is duplicated, so a large triangle (a ground quad) enlarges only its own leaf instead
-    of every leaf it straddles, which is what makes the reference's straddler-duplicating split
-    (bv_partition) cull poorly. Any covering membership gives the same hit results; this one is
-    the fast build for the workloads here, and the one splat_hitset_build_bv makes on the device.
-    Returns (bvs, memberships) like bv_partition."""
+    centroid, split at the median, 2**depth leaves in recursion order (siblings adjacent). A
+    node's triangles are ordered by (centroid on the split axis, index) and the first half goes
+    left, so equal centroids split by index. The split axis: for a node of 2..SAH_MAX triangles
+    the one whose halves have the least surface-area cost (area of the half's triangle bounds x
+    its count, summed over both halves; the lower axis on ties), above that the centroids' widest
+    axis. Every leaf sits at the same depth (the hitter's heap-indexed tree). Boxes may overlap
+    but nothing is duplicated, so a large triangle (a ground quad) enlarges only its own leaf
+    instead of every leaf it straddles, which is what makes the reference's straddler-duplicating
+    split (bv_partition) cull poorly. Any covering membership gives the same hit results; this
+    one is the fast build for the workloads here, and the one splat_hitset_build_bv makes on the
+    device. Returns (bvs, memberships) like bv_partition."""
     cent = centroids(ftris)
+    v = ftris["v"][:, 1:4, :].astype(np.float64)
+    tlo, thi = v.min(axis=1), v.max(axis=1)
 
     def rec(idx: np.ndarray, d: int) -> list[np.ndarray]:
         if d == 0 or idx.size <= 1:
             return [idx]
-        c = cent[idx]
-        ax = int(np.argmax(c.max(axis=0) - c.min(axis=0)))
-        o = idx[np.lexsort((idx, c[:, ax]))]
-        h = o.size // 2
+        h = idx.size // 2
+        if idx.size > SAH_MAX:
+            c = cent[idx]
+            ax = int(np.argmax(c.max(axis=0) - c.min(axis=0)))
+            o = idx[np.lexsort((idx, c[:, ax]))]
+        else:
+            o, best = None, None
+            for ax in range(3):
+                oa = idx[np.lexsort((idx, cent[idx, ax]))]
+                L, R = oa[:h], oa[h:]
+                cost = box_area(tlo[L].min(0), thi[L].max(0)) * L.size + box_area(tlo[R].min(0), thi[R].max(0)) * R.size
+                if best is None or cost < best:
+                    o, best = oa, cost
         return rec(o[:h], d - 1) + rec(o[h:], d - 1)
 
     parts = [p for p in rec(np.arange(1, ftris.shape[0], dtype=np.int64), depth) if p.size > 0]

@@ -9,7 +9,10 @@
 //     diagonals, the mean of the middle two for an even count), at most max_big of them, the
 //     largest first, ties to the lower index; they form the leading member list;
 //  3. the rest is split object-median `depth` times: a node's triangles are ordered by
-//     (centroid on its widest centroid axis, index) and the first half goes left. Each of three
+//     (centroid on its split axis, index) and the first half goes left. The split axis of a node
+//     of 2..SAH_MAX triangles has the least surface-area cost (area of each half's triangle
+//     bounds x its count, summed; the lower axis on ties), of a larger node the widest centroid
+//     extent. Each of three
 //     lists (one per axis) stays sorted by (centroid on that axis, index) within every node, so
 //     a node's widest axis is read off the ends of its segments, and its split is a stable
 //     partition of every list, done for all nodes of a level at once with one exclusive scan.
@@ -25,13 +28,15 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr int N_LISTS = 4;  // x, y, z centroid order; index order
+constexpr int32_t SAH_MAX = 4096;
 
 unsigned blocks(int64_t n) { return (unsigned)((n + BLK - 1) / BLK > 0 ? (n + BLK - 1) / BLK : 1); }
 
 // 1. per-triangle centroid (+0.0 makes -0 and +0 one key) and diagonal
 __global__ __launch_bounds__(BLK) void k_info(const splat_ftri_t* __restrict__ tris, int32_t n,
                                               double* __restrict__ cent, double* __restrict__ diag,
-                                              int32_t* __restrict__ ids, uint8_t* __restrict__ big) {
+                                              float* __restrict__ bounds, int32_t* __restrict__ ids,
+                                              uint8_t* __restrict__ big) {
   const int32_t t = blockIdx.x * BLK + threadIdx.x;
   if (t >= n) return;
   const splat_ftri_t& T = tris[t + 1];
@@ -40,6 +45,8 @@ __global__ __launch_bounds__(BLK) void k_info(const splat_ftri_t* __restrict__ t
     const double p = T.v[1][a], q = T.v[2][a], r = T.v[3][a];
     cent[(int64_t)a * n + t] = ((p + q) + r) / 3.0 + 0.0;
     const double hi = fmax(fmax(p, q), r), lo = fmin(fmin(p, q), r);
+    bounds[(int64_t)a * n + t] = (float)lo;  // the triangle's bounds: lo x, y, z then hi x, y, z
+    bounds[(int64_t)(3 + a) * n + t] = (float)hi;
     const double d = hi - lo;
     sq = a == 0 ? d * d : sq + d * d;
   }
@@ -118,16 +125,26 @@ __global__ __launch_bounds__(BLK) void k_compact(const int32_t* __restrict__ raw
   }
 }
 
-// 3. one level of the split. Per node: its widest centroid axis and half size; the children's
-// segments.
+// 3. one level of the split, one workgroup per node: its split axis and half size; the
+// children's segments.
+__device__ __forceinline__ double box_area(const float (&b)[6]) {
+  double d[3];
+  for (int c = 0; c < 3; ++c) {
+    const double e = (double)b[3 + c] - (double)b[c];
+    d[c] = e > 0.0 ? e : 0.0;
+  }
+  return 2.0 * ((d[0] * d[1] + d[1] * d[2]) + d[2] * d[0]);
+}
 __global__ __launch_bounds__(BLK) void k_nodes(const int32_t* __restrict__ seg, int32_t nodes, const int32_t* __restrict__ lists,
-                                               int32_t n, const double* __restrict__ cent, int32_t* __restrict__ axis,
-                                               int32_t* __restrict__ half, int32_t* __restrict__ seg_next) {
-  const int32_t k = blockIdx.x * BLK + threadIdx.x;
-  if (k >= nodes) return;
-  const int32_t s = seg[k], e = seg[k + 1], cnt = e - s;
+                                               int32_t n, const double* __restrict__ cent, const float* __restrict__ bounds,
+                                               int32_t* __restrict__ axis, int32_t* __restrict__ half,
+                                               int32_t* __restrict__ seg_next) {
+  __shared__ float part[BLK / 64][12];
+  const int32_t k = blockIdx.x;
+  const int32_t s = seg[k], e = seg[k + 1], cnt = e - s, h = cnt / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int32_t ax = 0;
-  if (cnt >= 2) {
+  if (cnt > SAH_MAX) {  // widest centroid extent (each list is sorted within the node)
     double best = -1.0;
     for (int a = 0; a < 3; ++a) {
       const int32_t* L = lists + (int64_t)a * n;
@@ -137,12 +154,56 @@ __global__ __launch_bounds__(BLK) void k_nodes(const int32_t* __restrict__ seg, 
         ax = a;
       }
     }
+  } else if (cnt >= 2) {  // least surface-area cost of the median split
+    double best = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const int32_t* L = lists + (int64_t)a * n;
+      // bounds of the left half [s, s + h) and the right half [s + h, e) in this axis' order
+      float b[12];
+      for (int c = 0; c < 3; ++c) {
+        b[c] = b[6 + c] = INFINITY;
+        b[3 + c] = b[9 + c] = -INFINITY;
+      }
+      for (int32_t p = s + threadIdx.x; p < e; p += BLK) {
+        const int32_t t = L[p];
+        const int o = p < s + h ? 0 : 6;
+        for (int c = 0; c < 3; ++c) {
+          b[o + c] = fminf(b[o + c], bounds[(int64_t)c * n + t]);
+          b[o + 3 + c] = fmaxf(b[o + 3 + c], bounds[(int64_t)(3 + c) * n + t]);
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1)
+        for (int j = 0; j < 12; ++j) {
+          const float o = __shfl_xor(b[j], off);
+          b[j] = (j % 6) < 3 ? fminf(b[j], o) : fmaxf(b[j], o);
+        }
+      if (lane == 0)
+        for (int j = 0; j < 12; ++j) part[wave][j] = b[j];
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int w = 1; w < BLK / 64; ++w)
+          for (int j = 0; j < 12; ++j) b[j] = (j % 6) < 3 ? fminf(b[j], part[w][j]) : fmaxf(b[j], part[w][j]);
+        float bl[6], br[6];
+        for (int j = 0; j < 6; ++j) {
+          bl[j] = b[j];
+          br[j] = b[6 + j];
+        }
+        const double cost = box_area(bl) * (double)h + box_area(br) * (double)(cnt - h);
+        if (a == 0 || cost < best) {
+          best = cost;
+          ax = a;
+        }
+      }
+      __syncthreads();
+    }
   }
-  axis[k] = ax;
-  half[k] = cnt / 2;
-  seg_next[2 * k] = s;
-  seg_next[2 * k + 1] = s + cnt / 2;
-  if (k == nodes - 1) seg_next[2 * nodes] = e;
+  if (threadIdx.x == 0) {
+    axis[k] = ax;
+    half[k] = h;
+    seg_next[2 * k] = s;
+    seg_next[2 * k + 1] = s + h;
+    if (k == nodes - 1) seg_next[2 * nodes] = e;
+  }
 }
 
 // side of every triangle at this level: right (1) when it is past the half of its node's order
@@ -204,6 +265,7 @@ __global__ __launch_bounds__(BLK) void k_ranges(const int32_t* __restrict__ seg,
 
 struct Scratch {
   double *cent, *keys_a, *keys_b;
+  float* bounds;
   int32_t *ids_a, *ids_b, *raw, *lists, *lists2, *big_list, *keep, *excl, *node_of, *node2, *seg, *seg2, *axis, *half,
       *summary;
   uint8_t *big, *right;
@@ -235,6 +297,7 @@ Scratch carve(void* base, int32_t n, int32_t depth) {
     return r;
   };
   S.cent = (double*)take(3 * nn * sizeof(double));
+  S.bounds = (float*)take(6 * nn * sizeof(float));
   S.keys_a = (double*)take(nn * sizeof(double));
   S.keys_b = (double*)take(nn * sizeof(double));
   S.ids_a = (int32_t*)take(nn * sizeof(int32_t));
@@ -267,7 +330,7 @@ int32_t bv_build_members(const splat_ftri_t* tris, int32_t n, int32_t depth, dou
                          void* scratch, int32_t* members, int2* ranges, int32_t* h_summary, hipStream_t s) {
   Scratch S = carve(scratch, n, depth);
   size_t cb = S.cub_bytes;
-  hipLaunchKernelGGL(k_info, dim3(blocks(n)), dim3(BLK), 0, s, tris, n, S.cent, S.keys_a, S.ids_a, S.big);
+  hipLaunchKernelGGL(k_info, dim3(blocks(n)), dim3(BLK), 0, s, tris, n, S.cent, S.keys_a, S.bounds, S.ids_a, S.big);
   if (hipcub::DeviceRadixSort::SortPairsDescending(S.cub, cb, S.keys_a, S.keys_b, S.ids_a, S.ids_b, n, 0, 64, s) !=
       hipSuccess)
     return SPLAT_ERR_HIP;
@@ -291,7 +354,8 @@ int32_t bv_build_members(const splat_ftri_t* tris, int32_t n, int32_t depth, dou
   int32_t *lists = S.lists, *lists2 = S.lists2, *node_of = S.node_of, *node2 = S.node2, *seg = S.seg, *seg2 = S.seg2;
   for (int l = 0; l < depth; ++l) {
     const int32_t nodes = 1 << l;
-    hipLaunchKernelGGL(k_nodes, dim3(blocks(nodes)), dim3(BLK), 0, s, seg, nodes, lists, n, S.cent, S.axis, S.half, seg2);
+    hipLaunchKernelGGL(k_nodes, dim3((unsigned)nodes), dim3(BLK), 0, s, seg, nodes, lists, n, S.cent, S.bounds, S.axis,
+                       S.half, seg2);
     hipLaunchKernelGGL(k_side, dim3(blocks(n)), dim3(BLK), 0, s, lists, n, S.summary, node_of, seg, S.axis, S.half,
                        S.right);
     hipLaunchKernelGGL(k_left, dim3(blocks(n4)), dim3(BLK), 0, s, lists, n, S.summary, S.right, S.keep);

@@ -22,11 +22,24 @@ def test_equal_centroids_split_by_index():
     assert [m.tolist() for m in mem] == [[2, 3], [4, 5], [6, 7], [8, 9]]
 
 
-def test_split_on_widest_axis_then_index():
+def test_split_axis_then_index():
     # centroids spread along y only; x ties everywhere: the split follows y
     verts = [[[0, 3 * k, 0], [1, 3 * k, 0], [0, 3 * k + 1, 0]] for k in (3, 0, 2, 1)]
     _, mem = bv.object_partition(_tris(verts), 1)
     assert [m.tolist() for m in mem] == [[3, 5], [2, 4]]
+
+
+def test_surface_area_axis_beats_widest_centroid_axis():
+    # two thin strips (flat in z, long in x) at y = 0 and y = 1; centroids spread 2 along x but
+    # only 1 along y. Splitting by x gives two 4 x 1 boxes (cost 2 x 8 x 2 = 32), by y two 6 x
+    # 0.01 slivers (cost 0.48): the surface-area rule splits by y, the widest centroid axis
+    # would split by x.
+    def strip(x, y):
+        return [[x - 2.0, y, 0.0], [x + 2.0, y, 0.0], [x, y + 0.01, 0.0]]
+    verts = [strip(0.0, 0.0), strip(2.0, 0.0), strip(0.0, 1.0), strip(2.0, 1.0)]
+    _, mem = bv.object_partition(_tris(verts), 1)
+    assert [m.tolist() for m in mem] == [[2, 3], [4, 5]]
+    assert bv.box_area(np.array([0.0, 0, 0]), np.array([4.0, 1, 0])) == 8.0
 
 
 def test_large_triangle_cap_by_index():
