@@ -65,7 +65,7 @@ __device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in
 // The bounce pass's LDS: one instance per workgroup, whichever bounce_tile variant runs. After the
 // tile's records are written out, `stage` holds the privatised light-map bins.
 struct BounceLDS {
-  uint2 stage[TILE * WORDS];  // 34,816 bytes: the tile's records
+  __align__(16) uint2 stage[TILE * WORDS];  // 34,816 bytes: the tile's records
   int16_t order[TILE];
   int32_t wave_a[TILE / 64], wave_r[TILE / 64];
   int32_t box[TILE / 64][4];  // per wave: min x, max x, min y, max y of its splat texels
@@ -157,8 +157,7 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
   if constexpr (CAM) {
     if (valid) *rec = raygen_ray(cam, base + tid);
   } else {
-    const uint2* src = reinterpret_cast<const uint2*>(in);
-    for (int e = threadIdx.x; e < nvalid * WORDS; e += TILE) stage[e] = src[e];
+    stage_words(stage, reinterpret_cast<const uint2*>(in), nvalid * WORDS);
   }
   __syncthreads();
 
@@ -224,14 +223,8 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
     hit_rest[off_r + pr + rr] = i;
   }
   __syncthreads();
-  uint2* dst_a = reinterpret_cast<uint2*>(out_active + off_a);
-  uint2* dst_r = reinterpret_cast<uint2*>(out_rest + off_r);
-  for (int e = threadIdx.x; e < nvalid * WORDS; e += TILE) {
-    const int slot = e / WORDS, w = e - slot * WORDS;
-    const uint2 v = stage[order[slot] * WORDS + w];
-    if (slot < na) dst_a[e] = v;
-    else dst_r[e - na * WORDS] = v;
-  }
+  write_words(stage, order, 0, na * WORDS, reinterpret_cast<uint2*>(out_active + off_a));
+  write_words(stage, order, na * WORDS, nvalid * WORDS, reinterpret_cast<uint2*>(out_rest + off_r));
   if (one_sample) splat_tile(lds, has, sx, sy, sv, P, lm);
   __syncthreads();  // the LDS stage is reused by the caller's next tile
 }
