@@ -33,31 +33,37 @@ __global__ __launch_bounds__(TILE) void k_count_active(const splat_adray_t* __re
 __global__ __launch_bounds__(1024) void k_scan_tiles(const int32_t* __restrict__ tile_counts, int64_t n_tiles,
                                                      const int64_t* __restrict__ d_count, int64_t R,
                                                      int32_t* __restrict__ tile_offsets, int64_t* __restrict__ d_total) {
-  __shared__ int64_t sums[1024];
+  __shared__ int64_t wave_sums[1024 / 64];
   if (d_count) {
     const int64_t A = *d_count < R ? *d_count : R;
     const int64_t live_tiles = (A + TILE - 1) / TILE;
     n_tiles = live_tiles < n_tiles ? live_tiles : n_tiles;
   }
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t chunk = (n_tiles + 1023) / 1024;
   const int64_t b = t * chunk, e = b + chunk < n_tiles ? b + chunk : n_tiles;
   int64_t s = 0;
   for (int64_t i = b; i < e; ++i) s += tile_counts[i];
-  sums[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    int64_t v = t >= off ? sums[t - off] : 0;
-    __syncthreads();
-    sums[t] += v;
-    __syncthreads();
+  // inclusive scan of the chunk sums: within each wave by shuffles, then over the 16 wave sums
+  int64_t incl = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t u = __shfl_up(incl, off);
+    if (lane >= off) incl += u;
   }
-  int64_t run = sums[t] - s;  // exclusive prefix of this chunk
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  int64_t before = 0, total = 0;
+  for (int w = 0; w < 1024 / 64; ++w) {
+    const int64_t ws = wave_sums[w];
+    before += w < wave ? ws : 0;
+    total += ws;
+  }
+  int64_t run = before + incl - s;  // exclusive prefix of this chunk
   for (int64_t i = b; i < e; ++i) {
     tile_offsets[i] = (int32_t)run;
     run += tile_counts[i];
   }
-  if (t == 1023) *d_total = sums[1023];
+  if (t == 0) *d_total = total;
 }
 
 __global__ __launch_bounds__(TILE) void k_scatter(const splat_adray_t* __restrict__ in, int64_t R,

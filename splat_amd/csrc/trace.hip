@@ -16,12 +16,14 @@ __global__ __launch_bounds__(TILE) void k_bounce_count(const splat_adray_t* __re
                                                        const splat_pinhole_t cam, int32_t from_cam,
                                                        int32_t* __restrict__ tile_counts) {
   const int64_t A = live_count_t(R, d_count);
-  const int64_t base = (int64_t)blockIdx.x * TILE;
-  if (base >= A) return;  // uniform over the workgroup
-  const int nvalid = (int)(A - base < TILE ? A - base : TILE);
-  const int c = from_cam ? bounce_count<true>(rays + base, keys + base, nvalid, P, true, cam, base)
-                         : bounce_count<false>(rays + base, keys + base, nvalid, P, true, cam, base);
-  if (threadIdx.x == 0) tile_counts[blockIdx.x] = c;
+  // a capped grid strides over the live tiles (the work per tile is even: its keys)
+  for (int64_t tile = blockIdx.x; tile * TILE < A; tile += gridDim.x) {  // uniform over the workgroup
+    const int64_t base = tile * TILE;
+    const int nvalid = (int)(A - base < TILE ? A - base : TILE);
+    const int c = from_cam ? bounce_count<true>(rays + base, keys + base, nvalid, P, true, cam, base)
+                           : bounce_count<false>(rays + base, keys + base, nvalid, P, true, cam, base);
+    if (threadIdx.x == 0) tile_counts[tile] = c;
+  }
 }
 
 // Start of a frame's loop: light map (when given) cleared, live count R, the first bounce's miss
@@ -190,14 +192,16 @@ __global__ void k_copy_rays(const uint2* __restrict__ src, uint2* __restrict__ d
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
+constexpr int64_t COUNT_GRID = 1024;  // workgroups of the count pass (about one resident set)
 void launch_bounce_count(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
                          const BounceParams& P, const splat_pinhole_t* cam, int32_t* tile_counts, hipStream_t s) {
   if (R <= 0) return;
   splat_pinhole_t c;
   if (cam) c = *cam;
   else std::memset(&c, 0, sizeof(c));
-  hipLaunchKernelGGL(k_bounce_count, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, keys,
-                     P, c, cam ? 1 : 0, tile_counts);
+  const int64_t tiles = (R + TILE - 1) / TILE;
+  hipLaunchKernelGGL(k_bounce_count, dim3((unsigned)(tiles < COUNT_GRID ? tiles : COUNT_GRID)), dim3(TILE), 0, s, rays,
+                     R, d_count, keys, P, c, cam ? 1 : 0, tile_counts);
 }
 static unsigned frame_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
