@@ -397,6 +397,43 @@ def test_trace_lights_vs_oracle(up, hitter_kind):
     assert want_tex.sum() > 0 and H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
 
 
+def test_trace_frame_graph_replays():
+    """The plain frame has no host synchronisation and no memset nodes, so a caller can capture
+    it in a HIP graph: every replay, with eager work and synchronisations between them, equals
+    the eager frame."""
+    import torch
+
+    sc = H.small_c2(2)
+    cam = scene.camera_c2(seed=6, height=96, width=96)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=5)
+    lm = abi.lightmap_params(128, 128)
+    tris_d = api.to_device(sc.ftris)
+    fr = api.Frame(cam, api.HIPBVHitter.build_on_device(tris_d, 5), tris_d, p, lm)
+    fr.trace()
+    _sync()
+    rays0, hit0, tex0 = fr.rays.clone(), fr.hit_idx.clone(), fr.tex.cpu().numpy()
+    side = torch.cuda.Stream()  # default priority
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fr.trace()  # warm-up on the capture's side stream
+    torch.cuda.current_stream().wait_stream(side)
+    _sync()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fr.trace()
+    for _ in range(3):
+        fr.rays.zero_()
+        fr.hit_idx.fill_(-1)
+        fr.tex.fill_(7.0)
+        _sync()
+        g.replay()
+        _sync()
+        assert torch.equal(fr.rays, rays0) and torch.equal(fr.hit_idx, hit0)
+        assert H.rel_l2(fr.tex.cpu().numpy(), tex0) < 1e-4
+        fr.trace()  # eager frames in between
+        _sync()
+
+
 def test_trace_frame_vs_oracle():
     sc = H.small_c2(2)
     cam = scene.camera_c2(seed=6, height=96, width=96)
