@@ -393,9 +393,11 @@ int32_t splat_shade(splat_adray_t* rays, int64_t R, const int32_t* hit_idx, cons
 /* ---- multi-frame engine: batching and continuous serving --------------------------------
  * Frames (each with its own resident scene, camera, depth and light map) occupy slots of an
  * engine. One step advances every admitted frame by one stage -- raygen + bounce 1, bounce k,
- * ..., final hit + splat -- in ONE fixed sequence of launches shared by all frames: the hitter
- * is a persistent kernel that pulls 64-ray work items of every frame from a device work queue.
- * A step has fixed launch shapes, so it is captured once into a HIP graph and replayed. Frames
+ * ..., final hit + splat -- in ONE fixed sequence of launches shared by all frames: a planning
+ * kernel turns the frames' device-side live counts into work-item prefixes, and every kernel
+ * maps its hardware-dispatched workgroups onto the items of all frames (the hitter's items are
+ * (64-ray group, split) pairs). A step has fixed launch shapes per hitter grid size, so it is
+ * captured once into a HIP graph and replayed. Frames
  * can be admitted between any two steps (continuous batching); each frame's result is the same
  * as splat_trace_frame's for the same request. The engine owns one device arena (allocated at
  * creation, the only allocation) from which each admitted frame takes its ray buffers;
@@ -423,6 +425,9 @@ typedef struct splat_frame_request {
 size_t splat_engine_frame_bytes(const splat_pinhole_t* cam, int32_t with_outputs);
 int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine_t** out);
 int32_t splat_engine_destroy(splat_engine_t* e);
+/* Takes a slot and arena space for the request; enqueues nothing: the next splat_engine_step
+ * uploads every request admitted since the previous step (a few copies per step, not per frame)
+ * and starts them. `stream` is not used. Steps of one engine go to one stream. */
 int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* req, void* stream);
 int32_t splat_engine_step(splat_engine_t* e, void* stream);
 /* Reports the tags of frames whose final step has completed (wait != 0: block until every

@@ -59,6 +59,7 @@ struct EngineDev {
   int64_t* counters;   // [slot][MAXD + 2]: live rays after bounce k (k = 0: R)
   int32_t* pre;        // [kind][MAX_SLOTS + 1] exclusive prefixes of work items
   int32_t* hit_split;  // [slot] splits per 64-ray group this step; negative: flat-scan tail
+  int32_t* admit;      // [MAX_SLOTS + 1]: count, then the slots admitted since the last step
   int32_t n_slots;
 };
 
@@ -78,6 +79,12 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
   __shared__ int32_t buf[MAX_SLOTS];
   const int t = threadIdx.x;
+  // frames admitted since the last step start at stage 0 (their slot records were uploaded
+  // with the step); the list is consumed
+  const int32_t n_admit = E.admit[0];
+  if (t < n_admit) E.stage[E.admit[1 + t]] = 0;
+  __syncthreads();
+  if (t == 0) E.admit[0] = 0;
   const bool in = t < E.n_slots;
   const int32_t s = in ? E.stage[t] : -1;
   int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0, 0};
@@ -398,6 +405,13 @@ struct splat_engine {
   SlotDev* h_slots = nullptr;  // pinned staging of the slot table
   splat_lightmap_params_t* h_lm = nullptr;
   int32_t* h_stage0 = nullptr;
+  // admissions are uploaded by the next step: the slot records [up_lo, up_hi] and the list of
+  // admitted slots (two pinned buffers, each reused once the copy that read it has completed)
+  std::vector<int32_t> admits;
+  int32_t up_lo = -1, up_hi = -1;
+  int32_t* h_admit[2] = {nullptr, nullptr};
+  hipEvent_t admit_ev[2] = {nullptr, nullptr};
+  int admit_cur = 0;
   std::vector<HostSlot> slots;
   std::deque<Pending> pending;
   std::vector<hipEvent_t> free_events;
@@ -448,12 +462,17 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   e->max_slots = max_slots;
   size_t dev_bytes = al(sizeof(SlotDev) * max_slots) + al(sizeof(splat_lightmap_params_t) * max_slots) +
                      al(sizeof(int32_t) * max_slots) + al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2)) +
-                     al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1)) + al(sizeof(int32_t) * MAX_SLOTS);
+                     al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1)) + al(sizeof(int32_t) * MAX_SLOTS) +
+                     al(sizeof(int32_t) * (MAX_SLOTS + 1));
   if (hipMalloc(&e->arena_base, (size_t)arena_bytes) != hipSuccess) { delete e; return SPLAT_ERR_BUDGET; }
   if (hipMalloc(&e->dev_mem, dev_bytes) != hipSuccess) { (void)hipFree(e->arena_base); delete e; return SPLAT_ERR_HIP; }
   if (hipHostMalloc(reinterpret_cast<void**>(&e->h_slots), sizeof(SlotDev) * max_slots) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&e->h_lm), sizeof(splat_lightmap_params_t) * max_slots) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&e->h_stage0), sizeof(int32_t) * max_slots) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&e->h_stage0), sizeof(int32_t) * max_slots) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&e->h_admit[0]), sizeof(int32_t) * (MAX_SLOTS + 1)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&e->h_admit[1]), sizeof(int32_t) * (MAX_SLOTS + 1)) != hipSuccess ||
+      hipEventCreateWithFlags(&e->admit_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->admit_ev[1], hipEventDisableTiming) != hipSuccess) {
     splat_engine_destroy(e);
     return SPLAT_ERR_HIP;
   }
@@ -464,10 +483,13 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   e->dev.stage = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * max_slots);
   e->dev.counters = reinterpret_cast<int64_t*>(p); p += al(sizeof(int64_t) * (size_t)max_slots * (MAXD + 2));
   e->dev.pre = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * N_KINDS * (MAX_SLOTS + 1));
-  e->dev.hit_split = reinterpret_cast<int32_t*>(p);
+  e->dev.hit_split = reinterpret_cast<int32_t*>(p); p += al(sizeof(int32_t) * MAX_SLOTS);
+  e->dev.admit = reinterpret_cast<int32_t*>(p);
   e->dev.n_slots = max_slots;
   for (int i = 0; i < max_slots; ++i) e->h_stage0[i] = -1;
-  if (hipMemcpy(e->dev.stage, e->h_stage0, sizeof(int32_t) * max_slots, hipMemcpyHostToDevice) != hipSuccess) {
+  e->h_admit[0][0] = 0;
+  if (hipMemcpy(e->dev.stage, e->h_stage0, sizeof(int32_t) * max_slots, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(e->dev.admit, e->h_admit[0], sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
     splat_engine_destroy(e);
     return SPLAT_ERR_HIP;
   }
@@ -494,6 +516,10 @@ int32_t splat_engine_destroy(splat_engine_t* e) {
   if (e->h_slots) (void)hipHostFree(e->h_slots);
   if (e->h_lm) (void)hipHostFree(e->h_lm);
   if (e->h_stage0) (void)hipHostFree(e->h_stage0);
+  for (int i = 0; i < 2; ++i) {
+    if (e->h_admit[i]) (void)hipHostFree(e->h_admit[i]);
+    if (e->admit_ev[i]) (void)hipEventDestroy(e->admit_ev[i]);
+  }
   if (e->dev_mem) (void)hipFree(e->dev_mem);
   if (e->arena_base) (void)hipFree(e->arena_base);
   delete e;
@@ -556,13 +582,10 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   D.optics = rq->optics;
   D.cam = cam;
   e->h_lm[slot] = lm;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (hipMemcpyAsync(e->dev.slots + slot, &D, sizeof(SlotDev), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(e->dev.lm + slot, e->h_lm + slot, sizeof(splat_lightmap_params_t), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(e->dev.stage + slot, e->h_stage0, sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess) {
-    for (const Block& b : got) e->arena.release(b.off, b.size);
-    return SPLAT_ERR_HIP;
-  }
+  (void)stream;  // nothing is enqueued here: the next step uploads the admissions
+  e->admits.push_back(slot);
+  e->up_lo = e->up_lo < 0 || slot < e->up_lo ? slot : e->up_lo;
+  e->up_hi = slot > e->up_hi ? slot : e->up_hi;
   hsl.used = true;
   hsl.stage = 0;
   hsl.depth = rq->depth;
@@ -583,6 +606,25 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
     groups += (h.R + HIT_ITEM - 1) / HIT_ITEM;
   }
   if (!any) return SPLAT_OK;
+  if (!e->admits.empty()) {
+    // the slot records of the frames admitted since the last step (one copy per table over the
+    // range they span: the other records in it are unchanged copies of their device records),
+    // then the list of those slots; k_plan starts them at stage 0
+    const int32_t lo = e->up_lo, n = e->up_hi - e->up_lo + 1;
+    int32_t* buf = e->h_admit[e->admit_cur];
+    if (hipEventSynchronize(e->admit_ev[e->admit_cur]) != hipSuccess) return SPLAT_ERR_HIP;  // its last copy is done
+    buf[0] = (int32_t)e->admits.size();
+    for (size_t i = 0; i < e->admits.size(); ++i) buf[1 + i] = e->admits[i];
+    if (hipMemcpyAsync(e->dev.slots + lo, e->h_slots + lo, sizeof(SlotDev) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(e->dev.lm + lo, e->h_lm + lo, sizeof(splat_lightmap_params_t) * n, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(e->dev.admit, buf, sizeof(int32_t) * (1 + e->admits.size()), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipEventRecord(e->admit_ev[e->admit_cur], s) != hipSuccess)
+      return SPLAT_ERR_HIP;
+    e->admit_cur ^= 1;
+    e->admits.clear();
+    e->up_lo = e->up_hi = -1;
+  }
   // hitter grid: a power of two of workgroups (4 waves each) that covers every item the live
   // counts can need, so a few graphs serve any mix of frames
   const int64_t need = (max_hit_items(groups) + 3) / 4;
