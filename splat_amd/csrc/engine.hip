@@ -275,29 +275,62 @@ __global__ __launch_bounds__(1024) void k_b_scan(EngineDev E) {
 }
 
 // bounce pass (bounce.hpp): evolve, splat the rays that become DIFFUSE, stable partition
-__global__ __launch_bounds__(TILE) void k_b_bounce(EngineDev E) {
+// One tile per workgroup (no loop: a grid-stride loop over the tiles made the compiler keep the
+// slot's parameters live across iterations, 188 VGPRs and 2 waves per SIMD). Slot fields read one
+// by one and the light-map parameters in LDS bring it to 128 VGPRs (4 waves; a few spilled
+// registers cost less than the lost wave). The grid is the step's hitter grid, which covers every
+// tile: tiles <= groups / 4 + live slots and hit_blocks >= (max(groups, 32768) + 8192) / 4 with at
+// most 1024 slots.
+__global__ __launch_bounds__(TILE, 4) void k_b_bounce(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
-  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
+  __shared__ splat_lightmap_params_t s_lm;  // the slot's light-map parameters
+  {
+    const int32_t i = blockIdx.x;
+    if (i >= P[n]) return;
     const int32_t s = item_slot(P, n, i);
-    const SlotDev& D = E.slots[s];
-    const int32_t st = E.stage[s];
-    const int64_t* C = E.counters + (int64_t)s * (MAXD + 2);
+    {
+      constexpr int NW = sizeof(splat_lightmap_params_t) / 4;
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(E.lm + s);
+      for (int w = threadIdx.x; w < NW; w += TILE) reinterpret_cast<uint32_t*>(&s_lm)[w] = src[w];
+      __syncthreads();
+    }
+    // the slot record and counters are read-only in this kernel: read field by field through the
+    // constant address space they are scalar loads into SGPRs rather than vector loads into VGPRs
+    const cptr<SlotDev> D = as_const(E.slots + s);
+    const int32_t st = *as_const(E.stage + s);
+    const cptr<int64_t> C = as_const(E.counters + (int64_t)s * (MAXD + 2));
     const int64_t A = C[st];
     const int32_t tile = i - P[s];
     const int64_t base = (int64_t)tile * TILE;
     const int nvalid = (int)(A - base < TILE ? A - base : TILE);
-    const int64_t off_a = D.tile_offsets[tile];
+    const int64_t off_a = D->tile_offsets[tile];
     const int64_t off_r = C[st + 1] + (base - off_a);
-    const BounceParams BP = slot_bounce(D, st + 1);
-    const uint64_t* keys = D.keys[st & 1] + base;
-    uint64_t* keys_next = D.keys[(st + 1) & 1];
-    if (st == 0)
-      bounce_tile<true>(nullptr, keys, keys_next, nvalid, BP, E.lm[s], D.cam, base, off_a, off_r, D.S[1], D.F,
-                        D.hit_final);
-    else
-      bounce_tile<false>(D.S[st & 1] + base, keys, keys_next, nvalid, BP, E.lm[s], D.cam, base, off_a, off_r,
-                         D.S[(st + 1) & 1], D.F, D.hit_final);
+    BounceParams BP;
+    BP.tris = D->tris;
+    BP.n_tris = D->n_tris;
+    BP.first_diffuse = D->first_diffuse;
+    BP.bounce = st + 1;
+    BP.force_rand = D->force_rand;
+    BP.seed = D->seed;
+    BP.optics = D->optics;
+    BP.tex = D->tex;
+    const bool odd = (st & 1) != 0;
+    splat_adray_t* const S0 = D->S[0];
+    splat_adray_t* const S1 = D->S[1];
+    uint64_t* const K0 = D->keys[0];
+    uint64_t* const K1 = D->keys[1];
+    const uint64_t* keys = (odd ? K1 : K0) + base;
+    uint64_t* keys_next = odd ? K0 : K1;
+    if (st == 0) {
+      const splat_pinhole_t cam = D->cam;
+      bounce_tile<true>(nullptr, keys, keys_next, nvalid, BP, s_lm, cam, base, off_a, off_r, S1, D->F,
+                        D->hit_final);
+    } else {
+      const splat_pinhole_t cam{};
+      bounce_tile<false>((odd ? S1 : S0) + base, keys, keys_next, nvalid, BP, s_lm, cam, base, off_a, off_r,
+                         odd ? S0 : S1, D->F, D->hit_final);
+    }
   }
 }
 
@@ -433,7 +466,7 @@ int32_t launch_step_kernels(splat_engine* e, int64_t hit_blocks, hipStream_t s) 
   hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_count, dim3(work), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
-  hipLaunchKernelGGL(k_b_bounce, dim3(work), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_bounce, dim3((unsigned)hit_blocks), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_final, dim3(work), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_finish, dim3(1), dim3(1024), 0, s, E);
   return hipGetLastError() == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
