@@ -49,8 +49,9 @@ __device__ __forceinline__ int64_t live_count(int64_t R, const int64_t* __restri
 // ---- bounding volumes -------------------------------------------------------------------------
 // The per-lane slab test against a conservative box (padded member bounds). Reciprocals are
 // precomputed per ray; |d| < 1e-30 is clamped so 0 * inf never appears.
-__device__ __forceinline__ bool slab_enter(const float4& lo, const float4& hi, const RayLanes& L, float ix, float iy,
-                                           float iz, float best) {
+// slab_entry also gives the entry distance tmin (>= 0).
+__device__ __forceinline__ bool slab_entry(const float4& lo, const float4& hi, const RayLanes& L, float ix, float iy,
+                                           float iz, float best, float& tmin_out) {
   float t1 = (lo.x - L.ox) * ix, t2 = (hi.x - L.ox) * ix;
   float tmin = fmaxf(0.0f, fminf(t1, t2)), tmax = fmaxf(t1, t2);
   t1 = (lo.y - L.oy) * iy;
@@ -61,7 +62,13 @@ __device__ __forceinline__ bool slab_enter(const float4& lo, const float4& hi, c
   t2 = (hi.z - L.oz) * iz;
   tmin = fmaxf(tmin, fminf(t1, t2));
   tmax = fminf(tmax, fmaxf(t1, t2));
+  tmin_out = tmin;
   return (tmin <= tmax) && (tmin <= best);
+}
+__device__ __forceinline__ bool slab_enter(const float4& lo, const float4& hi, const RayLanes& L, float ix, float iy,
+                                           float iz, float best) {
+  float tmin;
+  return slab_entry(lo, hi, L, ix, iy, iz, best, tmin);
 }
 
 // keep (t, idx) when it is a hit lexicographically before (best, bi); selects, not branches, so
@@ -79,11 +86,15 @@ __device__ __forceinline__ void take_nearer(bool h, float t, int32_t idx, float&
 __device__ __forceinline__ float bcast(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
-__device__ __forceinline__ void scan_records16(const TriRec* __restrict__ recs, int32_t cnt, const RayLanes& L,
-                                               float& best, int32_t& bi) {
+// (split in two so a caller can issue the loads of several leaves before testing any)
+__device__ __forceinline__ float4 load_records16(const TriRec* __restrict__ recs, int32_t cnt) {
   const int lane = threadIdx.x & 63;
   float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
   if ((lane >> 2) < cnt) v = reinterpret_cast<const float4*>(recs + (lane >> 2))[lane & 3];
+  return v;
+}
+__device__ __forceinline__ void test_records16(const float4& v, int32_t cnt, const RayLanes& L, float& best,
+                                               int32_t& bi) {
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     if (j < cnt) {
@@ -96,6 +107,10 @@ __device__ __forceinline__ void scan_records16(const TriRec* __restrict__ recs, 
       take_nearer(h, t, __float_as_int(q[1].w), best, bi);
     }
   }
+}
+__device__ __forceinline__ void scan_records16(const TriRec* __restrict__ recs, int32_t cnt, const RayLanes& L,
+                                               float& best, int32_t& bi) {
+  test_records16(load_records16(recs, cnt), cnt, L, best, bi);
 }
 
 struct HitStats {
@@ -116,9 +131,10 @@ __device__ __forceinline__ float key_t(uint64_t k) {
 // (a triangle held by two leaves ties with itself). MODE (BVMode) picks the top structure:
 //   BV_FLAT   -- every leaf box in order (the reference's one-level DPBVHitter);
 //   BV_GROUPS -- group boxes of gsize consecutive leaves, then their leaves;
-//   BV_TREE   -- depth-first over the binary tree, nearer child first along the node's split axis
-//                for the wave's first live ray, without a stack (heap indices: up = n / 2); a
-//                wave whose rays diverge in direction may walk per lane (tree_walk_lanes).
+//   BV_TREE   -- depth-first over the binary tree two levels at a time (a node's four
+//                grandchildren at once), nearer first along the split axes for the wave's first
+//                live ray; a wave whose rays diverge in direction may walk per lane
+//                (tree_walk_lanes).
 // Work split `split` of `n_split` takes groups/leaves split, split + n_split, ... (flat, groups)
 // or the subtree under node n_split + split (tree; n_split a power of two). Must be called by
 // whole waves.
@@ -264,6 +280,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     for (int32_t m = 0; m < rg.y; m += 16)
       scan_records16(S.recs + rg.x + m, rg.y - m < 16 ? rg.y - m : 16, L, best, bi);
   };
+
   // the records every ray tests before the structure (large triangles)
   if (S.n_top > 0) {
     const int32_t nt = S.n_top;
@@ -312,70 +329,105 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     if (live != 0 && split < n_sub && per_lane) {
       tree_walk_lanes<STATS>(L, valid, ix, iy, iz, S, n_sub + split, st, best, bi, shared_key);
     } else if (live != 0 && split < n_sub) {
-      // The wave walks one path: a node's two children are loaded together (one 64-byte scalar
-      // load) and both tested; it goes into the nearer entered child and marks the farther one
-      // pending when entered too; a pending sibling is re-tested (with the nearer hits found
-      // meanwhile) when the walk comes back to it. Bit d of `firsts` / `pending`: the node on the
-      // path at depth d below the split's root is the nearer child / its sibling is pending, so
-      // climbing needs no memory.
+      // The wave walks the binary tree two levels at a time. A node's four grandchildren are
+      // contiguous in the heap (4n .. 4n + 3: one 128-byte scalar load) and are tested at once: a
+      // parent's box is the union of its children's, so a ray entering a grandchild enters its
+      // parent too and the leaves reached are those of a level-by-level walk, with half the chain
+      // of dependent loads. Entered grandchildren go on a stack, farthest first for the wave's
+      // first live ray along the split axes (nearer child, then its nearer half: measured faster
+      // than ordering by that ray's entry distances), and are popped without a re-test. The stack holds wave-uniform node
+      // indices, one per lane of a VGPR: a step pops one entry and pushes at most four, over at
+      // most 12 steps (24 levels), so it never holds more than 37 of the 64 lanes. Leaves are
+      // scanned as soon as they are reached, nearer first, each re-tested with the hits found
+      // meanwhile.
       const int32_t root = n_sub + split;
-      bool lane_in = false;  // this lane entered the current node's box
-      auto test = [&](const float4& lo, const float4& hi) {
-        lane_in = false;
-        if (__float_as_int(lo.w) & TREE_EMPTY) return false;
-        lane_in = valid && slab_enter(lo, hi, L, ix, iy, iz, best);
-        if constexpr (STATS) n_box += valid ? 1ull : 0ull;
-        return __builtin_amdgcn_ballot_w64(lane_in) != 0;
+      int32_t stack = 0, sp = 0;  // lane k of `stack`: entry k
+      const int32_t my_lane = threadIdx.x & 63;
+      auto push = [&](int32_t v) {
+        stack = my_lane == sp ? v : stack;
+        ++sp;
       };
-      int32_t n = root, depth = 0;
-      uint32_t firsts = 0, pending = 0;
-      float4 nlo = nodes[2 * root];
-      bool go = test(nlo, nodes[2 * root + 1]);
-      while (go) {  // some lane entered node n
-        if (n >= leaf0) {
-          const uint64_t k = fetch();
-          scan_members(n - leaf0, lane_in);
-          merge(k);
-        } else {
-          const int32_t code = __float_as_int(nlo.w);
-          const int32_t near = ((neg >> (code & TREE_AXIS)) & 1) ^ ((code & TREE_SWAP) ? 1 : 0);
-          const float4 alo = nodes[4 * n], ahi = nodes[4 * n + 1], blo = nodes[4 * n + 2], bhi = nodes[4 * n + 3];
-          const bool in_a = test(alo, ahi);
-          const bool lane_a = lane_in;
-          const bool in_b = test(blo, bhi);
-          const bool lane_b = lane_in;
-          const bool in_near = near ? in_b : in_a, in_far = near ? in_a : in_b;
-          if (in_near || in_far) {
-            ++depth;
-            const uint32_t bit = 1u << depth;
-            const bool take_b = in_near ? near != 0 : near == 0;
-            n = 2 * n + (take_b ? 1 : 0);
-            nlo = take_b ? blo : alo;
-            lane_in = take_b ? lane_b : lane_a;
-            if (in_near) firsts |= bit;
-            else firsts &= ~bit;
-            if (in_near && in_far) pending |= bit;
-            else pending &= ~bit;
-            continue;
-          }
+      // entry distance of this lane's ray into a node box: tmin where the slab intervals overlap,
+      // NaN where they do not (or the lane is idle, or the subtree is empty), so `t <= best` is
+      // the slab test, and stays exact later for a nearer best (tmin and tmax do not depend on it)
+      auto lane_t = [&](const float4& lo, const float4& hi) -> float {
+        if (!valid || (__float_as_int(lo.w) & TREE_EMPTY)) return __builtin_nanf("");
+        if constexpr (STATS) n_box += 1ull;
+        float tmin;
+        return slab_entry(lo, hi, L, ix, iy, iz, INFINITY, tmin) ? tmin : __builtin_nanf("");
+      };
+      // the child of a node with split code c that the wave's first live ray meets first (0 or 1):
+      // the lower one along the split axis when the ray points up it
+      auto near_half = [&](int32_t c) { return ((neg >> (c & TREE_AXIS)) & 1) ^ ((c & TREE_SWAP) ? 1 : 0); };
+      auto leaf = [&](int32_t m, bool e) {
+        const uint64_t k = fetch();
+        scan_members(m - leaf0, e);
+        merge(k);
+      };
+      // leaf m whose box this lane entered at distance t, unless the hits found since rule it out
+      auto visit = [&](int32_t m, float t) {
+        const bool e = t <= best;
+        if (__builtin_amdgcn_ballot_w64(e) != 0) leaf(m, e);
+      };
+      {
+        const float t = lane_t(nodes[2 * root], nodes[2 * root + 1]);
+        if (__builtin_amdgcn_ballot_w64(t <= best) != 0) {
+          if (root >= leaf0) visit(root, t);
+          else push(root);
         }
-        // climb to the nearest pending sibling on the path and re-test it
-        go = false;
-        while (depth > 0) {
-          const uint32_t bit = 1u << depth;
-          if (firsts & pending & bit) {
-            firsts &= ~bit;
-            pending &= ~bit;
-            n ^= 1;
-            nlo = nodes[2 * n];
-            if (test(nlo, nodes[2 * n + 1])) {
-              go = true;
-              break;
-            }
-            continue;  // now a far child: the next iteration climbs past it
-          }
-          n >>= 1;
-          --depth;
+      }
+      while (sp > 0) {
+        const int32_t n = __builtin_amdgcn_readlane(stack, --sp);
+        if (2 * n >= leaf0) {  // the children are leaves: nearer first
+          const float ta = lane_t(nodes[4 * n], nodes[4 * n + 1]);
+          const float tb = lane_t(nodes[4 * n + 2], nodes[4 * n + 3]);
+          const bool wa = __builtin_amdgcn_ballot_w64(ta <= best) != 0, wb = __builtin_amdgcn_ballot_w64(tb <= best) != 0;
+          const bool b_first = wb && (!wa || near_half(__float_as_int(nodes[2 * n].w)) == 1);
+          visit(2 * n + (b_first ? 1 : 0), b_first ? tb : ta);
+          if (wa && wb) visit(2 * n + (b_first ? 0 : 1), b_first ? ta : tb);
+          continue;
+        }
+        // named values, no arrays: a select between array elements puts the array in scratch
+        const int32_t g = 8 * n;
+        const float t0 = lane_t(nodes[g], nodes[g + 1]), t1 = lane_t(nodes[g + 2], nodes[g + 3]),
+                    t2 = lane_t(nodes[g + 4], nodes[g + 5]), t3 = lane_t(nodes[g + 6], nodes[g + 7]);
+        const bool w0 = __builtin_amdgcn_ballot_w64(t0 <= best) != 0, w1 = __builtin_amdgcn_ballot_w64(t1 <= best) != 0,
+                   w2 = __builtin_amdgcn_ballot_w64(t2 <= best) != 0, w3 = __builtin_amdgcn_ballot_w64(t3 <= best) != 0;
+        const uint32_t wmask = (w0 ? 1u : 0u) | (w1 ? 2u : 0u) | (w2 ? 4u : 0u) | (w3 ? 8u : 0u);
+        if (wmask == 0) continue;
+        // visiting rank of grandchild j = 2c + h (child c, half h): nearer child first, then its
+        // nearer half, by the split codes of n and of its two children (loaded with the boxes)
+        const int32_t cn = __float_as_int(nodes[2 * n].w), ca = __float_as_int(nodes[4 * n].w),
+                      cb = __float_as_int(nodes[4 * n + 2].w);
+        const int32_t nn = near_half(cn), na = near_half(ca), nb = near_half(cb);
+        auto rank = [&](int32_t j) {
+          const int32_t c = j >> 1, h = j & 1;
+          return (c == nn ? 0 : 2) + (h == (c ? nb : na) ? 0 : 1);
+        };
+        int32_t k0 = w0 ? rank(0) : INT32_MAX, k1 = w1 ? rank(1) : INT32_MAX,
+                k2 = w2 ? rank(2) : INT32_MAX, k3 = w3 ? rank(3) : INT32_MAX;
+        int32_t i0 = 0, i1 = 1, i2 = 2, i3 = 3;
+        // sort (key, index) ascending: a network of five compare-exchanges
+        auto cx = [](int32_t& ka, int32_t& ia, int32_t& kb, int32_t& ib) {
+          const bool sw = kb < ka;
+          const int32_t k = ka, i = ia;
+          ka = sw ? kb : ka; ia = sw ? ib : ia;
+          kb = sw ? k : kb; ib = sw ? i : ib;
+        };
+        cx(k0, i0, k1, i1); cx(k2, i2, k3, i3); cx(k0, i0, k2, i2); cx(k1, i1, k3, i3); cx(k1, i1, k2, i2);
+        if (4 * n >= leaf0) {  // the grandchildren are leaves: scan them nearest first
+          auto pick = [](int32_t j, float a0, float a1, float a2, float a3) {
+            return j == 0 ? a0 : j == 1 ? a1 : j == 2 ? a2 : a3;
+          };
+          if ((wmask >> i0) & 1u) visit(4 * n + i0, pick(i0, t0, t1, t2, t3));
+          if ((wmask >> i1) & 1u) visit(4 * n + i1, pick(i1, t0, t1, t2, t3));
+          if ((wmask >> i2) & 1u) visit(4 * n + i2, pick(i2, t0, t1, t2, t3));
+          if ((wmask >> i3) & 1u) visit(4 * n + i3, pick(i3, t0, t1, t2, t3));
+        } else {
+          if ((wmask >> i3) & 1u) push(4 * n + i3);
+          if ((wmask >> i2) & 1u) push(4 * n + i2);
+          if ((wmask >> i1) & 1u) push(4 * n + i1);
+          if ((wmask >> i0) & 1u) push(4 * n + i0);
         }
       }
     }
