@@ -63,15 +63,19 @@ struct EngineDev {
   int32_t n_slots;
 };
 
+// The slot of work item i: the largest s with pre[s] <= i (slots with no items share their
+// successor's prefix; pre[0] = 0 <= i < pre[n]). The prefixes do not decrease, so that s is the
+// number of s' in [1, n) with pre[s'] <= i: the wave compares 64 prefixes per (independent) load
+// and counts with a ballot, instead of a binary search's chain of dependent loads. Called by whole
+// waves with i the same on every lane.
 __device__ __forceinline__ int32_t item_slot(const int32_t* __restrict__ pre, int32_t n, int32_t i) {
-  // largest s with pre[s] <= i (slots with no items share their successor's prefix)
-  int32_t lo = 0, hi = n;  // pre[n] = total > i
-  while (hi - lo > 1) {
-    int32_t mid = (lo + hi) >> 1;
-    if (pre[mid] <= i) lo = mid;
-    else hi = mid;
+  const int32_t lane = threadIdx.x & 63;
+  int32_t s = 0;
+  for (int32_t b = 1; b < n; b += 64) {
+    const int32_t k = b + lane;
+    s += __builtin_popcountll(__builtin_amdgcn_ballot_w64(k < n && pre[k] <= i));
   }
-  return lo;
+  return s;
 }
 __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -153,9 +157,18 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
   const int32_t* Q = E.pre + K_CLEAR * (MAX_SLOTS + 1);
   for (int32_t i = blockIdx.x; i < Q[n]; i += gridDim.x) {
     const int32_t s = item_slot(Q, n, i);
-    const SlotDev& D = E.slots[s];
+    // the slot's fields in registers: read through the (unwritten) record each iteration, the
+    // float stores below could alias them and force a reload per store
+    const cptr<SlotDev> D = as_const(E.slots + s);
+    float* const tex = D->tex;
+    const int64_t tex_n = D->tex_n;
     const int64_t b = (int64_t)(i - Q[s]) * CLEAR_ITEM;
-    for (int64_t k = b + threadIdx.x; k < b + CLEAR_ITEM && k < D.tex_n; k += blockDim.x) D.tex[k] = 0.0f;
+    if (b + CLEAR_ITEM <= tex_n && (reinterpret_cast<uintptr_t>(tex) & 15) == 0) {
+      float4* const t4 = reinterpret_cast<float4*>(tex + b);
+      for (int k = threadIdx.x; k < CLEAR_ITEM / 4; k += blockDim.x) t4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      for (int64_t k = b + threadIdx.x; k < b + CLEAR_ITEM && k < tex_n; k += blockDim.x) tex[k] = 0.0f;
+    }
   }
   // keys of a new frame's rays <- (Inf, 1) before the hitter merges into them (later stages get
   // theirs from the bounce pass)
@@ -163,10 +176,10 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
   constexpr int PER_BLOCK = 256 / HIT_ITEM;
   for (int32_t i = blockIdx.x * PER_BLOCK + threadIdx.x / HIT_ITEM; i < H[n]; i += gridDim.x * PER_BLOCK) {
     const int32_t s = item_slot(H, n, i);
-    if (E.stage[s] != 0) continue;
-    const SlotDev& D = E.slots[s];
+    if (*as_const(E.stage + s) != 0) continue;
+    const cptr<SlotDev> D = as_const(E.slots + s);
     const int64_t r = (int64_t)(i - H[s]) * HIT_ITEM + (threadIdx.x & (HIT_ITEM - 1));
-    if (r < D.R) D.keys[0][r] = KEY_MISS;
+    if (r < D->R) D->keys[0][r] = KEY_MISS;
   }
 }
 
