@@ -8,7 +8,8 @@
 // A step is one fixed sequence of launches for every slot at once: a planning kernel turns the
 // device-side live counts into per-kind work-item prefixes (and the hitter's split factor); the
 // hitter runs one wave per (64-ray group, split) item of any frame, so frames of any size and
-// scene cost share the chip; the other stages walk their items grid-stride. Launch shapes depend
+// scene cost share the chip; the bounce pass takes one tile per workgroup over the hitter's grid,
+// the other stages walk their items grid-stride. Launch shapes depend
 // only on the hitter's grid size class, so each class is captured into a HIP graph once and
 // replayed. The host tracks stages itself (every admitted frame advances exactly one stage per
 // step), so it never reads device state to learn that a frame has finished: an event recorded
