@@ -36,20 +36,28 @@ struct BounceParams {
 
 __device__ __forceinline__ int32_t clamp_index(int32_t i, int32_t N) { return (i < 1 || i > N) ? 1 : i; }
 
-// ACTIVE rays after this bounce among the tile's first nvalid rays (a whole TILE-thread workgroup
-// calls it; thread 0 gets the count). all_active: every ray of the tile is known to be ACTIVE
-// (the loop's live rays), so no ray is read unless the sentinel test needs it. CAM: the rays are
-// the pinhole's rays base, base + 1, ... (not yet in memory).
+// ACTIVE rays after this bounce among the tile's first nvalid rays, counted by one wave (every
+// lane returns the count): lane l takes rays l, l + 64, l + 128, l + 192, whose key loads are
+// independent, so a wave moves a whole tile's keys per memory round trip. all_active: every ray of
+// the tile is known to be ACTIVE (the loop's live rays), so no ray is read unless the sentinel
+// test needs it. CAM: the rays are the pinhole's rays base, base + 1, ... (not yet in memory).
 template <bool CAM>
 __device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in, const uint64_t* __restrict__ keys,
                                             int nvalid, const BounceParams& P, bool all_active,
                                             const splat_pinhole_t& cam, int64_t base) {
-  bool act = false;
-  const int t = threadIdx.x;
-  if (t < nvalid && (CAM || all_active || in[t].status == SPLAT_RAY_ACTIVE)) {
-    const int32_t i = clamp_index(key_index(keys[t]), P.n_tris);
+  static_assert(TILE == 256, "four rays per lane");
+  const int lane = threadIdx.x & 63;
+  uint64_t k0 = KEY_MISS, k1 = KEY_MISS, k2 = KEY_MISS, k3 = KEY_MISS;
+  if (lane < nvalid) k0 = keys[lane];
+  if (lane + 64 < nvalid) k1 = keys[lane + 64];
+  if (lane + 128 < nvalid) k2 = keys[lane + 128];
+  if (lane + 192 < nvalid) k3 = keys[lane + 192];
+  const bool live = P.first_diffuse > 1 && sentinel_live(P.tris);  // uniform
+  auto active = [&](int t, uint64_t key) {
+    if (t >= nvalid || !(CAM || all_active || in[t].status == SPLAT_RAY_ACTIVE)) return false;
+    const int32_t i = clamp_index(key_index(key), P.n_tris);
     bool tri1 = false;
-    if (i == 1 && P.first_diffuse > 1 && sentinel_live(P.tris)) {
+    if (i == 1 && live) {
       if constexpr (CAM) {
         const splat_adray_t g = raygen_ray(cam, base + t);
         tri1 = sentinel_hit(P.tris, ld3(g.pos), ld3(g.dir), g.ignore_tri);
@@ -57,9 +65,12 @@ __device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in
         tri1 = sentinel_hit(P.tris, ld3(in[t].pos), ld3(in[t].dir), in[t].ignore_tri);
       }
     }
-    act = status_after(i, P.first_diffuse, tri1) == SPLAT_RAY_ACTIVE;
-  }
-  return __syncthreads_count(act);
+    return status_after(i, P.first_diffuse, tri1) == SPLAT_RAY_ACTIVE;
+  };
+  return __builtin_popcountll(__builtin_amdgcn_ballot_w64(active(lane, k0))) +
+         __builtin_popcountll(__builtin_amdgcn_ballot_w64(active(lane + 64, k1))) +
+         __builtin_popcountll(__builtin_amdgcn_ballot_w64(active(lane + 128, k2))) +
+         __builtin_popcountll(__builtin_amdgcn_ballot_w64(active(lane + 192, k3)));
 }
 
 // The bounce pass's LDS: one instance per workgroup, whichever bounce_tile variant runs. After the

@@ -177,7 +177,8 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
   constexpr int PER_BLOCK = 256 / HIT_ITEM;
   for (int32_t i = blockIdx.x * PER_BLOCK + threadIdx.x / HIT_ITEM; i < H[n]; i += gridDim.x * PER_BLOCK) {
     const int32_t s = item_slot(H, n, i);
-    if (*as_const(E.stage + s) != 0) continue;
+    // a frame hit without splits has every key stored by the hitter, not merged
+    if (*as_const(E.stage + s) != 0 || abs(*as_const(E.hit_split + s)) == 1) continue;
     const cptr<SlotDev> D = as_const(E.slots + s);
     const int64_t r = (int64_t)(i - H[s]) * HIT_ITEM + (threadIdx.x & (HIT_ITEM - 1));
     if (r < D->R) D->keys[0][r] = KEY_MISS;
@@ -224,37 +225,40 @@ __global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
   }
 }
 
-__device__ __forceinline__ BounceParams slot_bounce(const SlotDev& D, int32_t bounce) {
-  BounceParams P;
-  P.tris = D.tris;
-  P.n_tris = D.n_tris;
-  P.first_diffuse = D.first_diffuse;
-  P.bounce = bounce;
-  P.force_rand = D.force_rand;
-  P.seed = D.seed;
-  P.optics = D.optics;
-  P.tex = D.tex;
-  return P;
-}
-
-// count pass of the fused bounce (bounce.hpp): ACTIVE rays per tile from the hit keys
+// count pass of the fused bounce (bounce.hpp): ACTIVE rays per tile from the hit keys, one tile
+// per wave (its grid holds four waves per tile-count bound: hit_blocks >= the tile count)
 __global__ __launch_bounds__(TILE) void k_b_count(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
-  for (int32_t i = blockIdx.x; i < P[n]; i += gridDim.x) {
-    const int32_t s = item_slot(P, n, i);
-    const SlotDev& D = E.slots[s];
-    const int32_t st = E.stage[s];
-    const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
-    const int32_t tile = i - P[s];
-    const int64_t base = (int64_t)tile * TILE;
-    const int nvalid = (int)(A - base < TILE ? A - base : TILE);
-    const BounceParams BP = slot_bounce(D, st + 1);
-    const uint64_t* keys = D.keys[st & 1] + base;
-    const int c = st == 0 ? bounce_count<true>(nullptr, keys, nvalid, BP, true, D.cam, base)
-                          : bounce_count<false>(D.S[st & 1] + base, keys, nvalid, BP, true, D.cam, base);
-    if (threadIdx.x == 0) D.tile_counts[tile] = c;
+  const int32_t i = (int32_t)blockIdx.x * (TILE / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (i >= P[n]) return;  // whole waves
+  const int32_t s = item_slot(P, n, i);
+  const cptr<SlotDev> D = as_const(E.slots + s);
+  const int32_t st = *as_const(E.stage + s);
+  const int64_t A = *as_const(E.counters + (int64_t)s * (MAXD + 2) + st);
+  const int32_t tile = i - P[s];
+  const int64_t base = (int64_t)tile * TILE;
+  const int nvalid = (int)(A - base < TILE ? A - base : TILE);
+  BounceParams BP;
+  BP.tris = D->tris;
+  BP.n_tris = D->n_tris;
+  BP.first_diffuse = D->first_diffuse;
+  BP.bounce = st + 1;
+  BP.force_rand = D->force_rand;
+  BP.seed = D->seed;
+  BP.optics = D->optics;
+  BP.tex = D->tex;
+  const bool odd = (st & 1) != 0;
+  const uint64_t* keys = (odd ? D->keys[1] : D->keys[0]) + base;
+  int c;
+  if (st == 0) {
+    const splat_pinhole_t cam = D->cam;
+    c = bounce_count<true>(nullptr, keys, nvalid, BP, true, cam, base);
+  } else {
+    const splat_pinhole_t cam{};
+    c = bounce_count<false>((odd ? D->S[1] : D->S[0]) + base, keys, nvalid, BP, true, cam, base);
   }
+  if ((threadIdx.x & 63) == 0) D->tile_counts[tile] = c;
 }
 
 // one workgroup per slot: exclusive scan of its live tiles' ACTIVE counts
@@ -472,13 +476,13 @@ namespace {
 
 int32_t launch_step_kernels(splat_engine* e, int64_t hit_blocks, hipStream_t s) {
   const EngineDev& E = e->dev;
-  // the tile-item stages get about one workgroup per tile (hit_blocks >= the tile count), so
-  // workgroups at different phases of their tile overlap on a CU as in the single-frame passes
+  // hit_blocks >= the tile count: the bounce pass takes one tile per workgroup over it and the
+  // count pass one tile per wave over a quarter of it; the other stages stride over a capped grid
   const unsigned work = (unsigned)(hit_blocks < e->grid ? e->grid : (hit_blocks > 65536 ? 65536 : hit_blocks));
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
   hipLaunchKernelGGL(k_b_raygen, dim3(work), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
-  hipLaunchKernelGGL(k_b_count, dim3(work), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_count, dim3((unsigned)((hit_blocks + 3) / 4)), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
   hipLaunchKernelGGL(k_b_bounce, dim3((unsigned)hit_blocks), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_final, dim3(work), dim3(TILE), 0, s, E);

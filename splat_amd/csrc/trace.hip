@@ -16,13 +16,14 @@ __global__ __launch_bounds__(TILE) void k_bounce_count(const splat_adray_t* __re
                                                        const splat_pinhole_t cam, int32_t from_cam,
                                                        int32_t* __restrict__ tile_counts) {
   const int64_t A = live_count_t(R, d_count);
-  // a capped grid strides over the live tiles (the work per tile is even: its keys)
-  for (int64_t tile = blockIdx.x; tile * TILE < A; tile += gridDim.x) {  // uniform over the workgroup
+  // one tile per wave; a capped grid strides over the live tiles (the work per tile is even: its keys)
+  const int64_t wave = (int64_t)blockIdx.x * (TILE / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int64_t tile = wave; tile * TILE < A; tile += (int64_t)gridDim.x * (TILE / 64)) {  // uniform over the wave
     const int64_t base = tile * TILE;
     const int nvalid = (int)(A - base < TILE ? A - base : TILE);
     const int c = from_cam ? bounce_count<true>(rays + base, keys + base, nvalid, P, true, cam, base)
                            : bounce_count<false>(rays + base, keys + base, nvalid, P, true, cam, base);
-    if (threadIdx.x == 0) tile_counts[tile] = c;
+    if ((threadIdx.x & 63) == 0) tile_counts[tile] = c;
   }
 }
 
@@ -199,8 +200,8 @@ void launch_bounce_count(const splat_adray_t* rays, int64_t R, const int64_t* d_
   splat_pinhole_t c;
   if (cam) c = *cam;
   else std::memset(&c, 0, sizeof(c));
-  const int64_t tiles = (R + TILE - 1) / TILE;
-  hipLaunchKernelGGL(k_bounce_count, dim3((unsigned)(tiles < COUNT_GRID ? tiles : COUNT_GRID)), dim3(TILE), 0, s, rays,
+  const int64_t groups = (R + 4 * TILE - 1) / (4 * TILE);  // four tiles per workgroup
+  hipLaunchKernelGGL(k_bounce_count, dim3((unsigned)(groups < COUNT_GRID ? groups : COUNT_GRID)), dim3(TILE), 0, s, rays,
                      R, d_count, keys, P, c, cam ? 1 : 0, tile_counts);
 }
 static unsigned frame_grid(int64_t n) {
