@@ -30,7 +30,9 @@ namespace spl {
 
 constexpr int MAX_SLOTS = 1024;
 constexpr int MAXD = SPLAT_ENGINE_MAX_DEPTH;
-enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, K_WAVE, N_KINDS };  // K_HIT: 64-ray groups; K_WAVE: hitter waves
+// K_HIT: 64-ray groups; K_WAVE: hitter waves; K_KEYS: the 64-ray groups whose miss keys are filled
+// before the first hit (new frames hit with splits, which merge into the keys)
+enum Kind { K_RAYGEN = 0, K_CLEAR, K_HIT, K_TILE, K_FINAL, K_WAVE, K_KEYS, N_KINDS };
 constexpr int HIT_ITEM = 64;     // rays per hitter work item (one wave)
 constexpr int CLEAR_ITEM = 4096;  // light-map floats per clear item
 
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
   if (t == 0) E.admit[0] = 0;
   const bool in = t < E.n_slots;
   const int32_t s = in ? E.stage[t] : -1;
-  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0, 0};
+  int32_t cnt[N_KINDS] = {0, 0, 0, 0, 0, 0, 0};
   bool tail = false;
   int32_t leaves = 1;
   if (s >= 0) {
@@ -130,6 +132,7 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
   const int32_t S_slot = tail ? tail_split_factor(g_tail, leaves) : S_walk;
   if (in) E.hit_split[t] = tail ? -S_slot : S_slot;
   cnt[K_WAVE] = cnt[K_HIT] * S_slot;
+  cnt[K_KEYS] = s == 0 && S_slot > 1 ? cnt[K_HIT] : 0;  // without splits the hitter stores every key
   for (int k = 0; k < N_KINDS; ++k) {
     buf[t] = cnt[k];
     __syncthreads();
@@ -171,14 +174,12 @@ __global__ __launch_bounds__(256) void k_b_raygen(EngineDev E) {
       for (int64_t k = b + threadIdx.x; k < b + CLEAR_ITEM && k < tex_n; k += blockDim.x) tex[k] = 0.0f;
     }
   }
-  // keys of a new frame's rays <- (Inf, 1) before the hitter merges into them (later stages get
-  // theirs from the bounce pass)
-  const int32_t* H = E.pre + K_HIT * (MAX_SLOTS + 1);
+  // keys of a new frame's rays <- (Inf, 1) before the hitter's splits merge into them (later
+  // stages get theirs from the bounce pass)
+  const int32_t* H = E.pre + K_KEYS * (MAX_SLOTS + 1);
   constexpr int PER_BLOCK = 256 / HIT_ITEM;
   for (int32_t i = blockIdx.x * PER_BLOCK + threadIdx.x / HIT_ITEM; i < H[n]; i += gridDim.x * PER_BLOCK) {
     const int32_t s = item_slot(H, n, i);
-    // a frame hit without splits has every key stored by the hitter, not merged
-    if (*as_const(E.stage + s) != 0 || abs(*as_const(E.hit_split + s)) == 1) continue;
     const cptr<SlotDev> D = as_const(E.slots + s);
     const int64_t r = (int64_t)(i - H[s]) * HIT_ITEM + (threadIdx.x & (HIT_ITEM - 1));
     if (r < D->R) D->keys[0][r] = KEY_MISS;
@@ -466,26 +467,31 @@ struct splat_engine {
   std::vector<HostSlot> slots;
   std::deque<Pending> pending;
   std::vector<hipEvent_t> free_events;
-  std::map<int64_t, hipGraphExec_t> graphs;  // one captured step per hitter grid size
+  std::map<int64_t, hipGraphExec_t> graphs;  // one captured step per set of grid sizes (StepGrids::key)
   hipStream_t cap_stream = nullptr;  // graphs are captured here (the null stream cannot capture)
   int use_graph = 1;
-  int grid = 1024;
 };
 
 namespace {
 
-int32_t launch_step_kernels(splat_engine* e, int64_t hit_blocks, hipStream_t s) {
+// Grids of one step: hit_blocks >= the tile count, so the bounce pass takes one tile per workgroup
+// over it and the count pass one tile per wave over a quarter of it; the first-stage and last-stage
+// kernels stride over grids sized by the host for the frames at those stages (1 when none is).
+struct StepGrids {
+  int64_t hit_blocks, first_blocks, final_blocks;
+  int64_t key() const { return hit_blocks | (first_blocks << 25) | (final_blocks << 42); }
+};
+
+int32_t launch_step_kernels(splat_engine* e, const StepGrids& g, hipStream_t s) {
   const EngineDev& E = e->dev;
-  // hit_blocks >= the tile count: the bounce pass takes one tile per workgroup over it and the
-  // count pass one tile per wave over a quarter of it; the other stages stride over a capped grid
-  const unsigned work = (unsigned)(hit_blocks < e->grid ? e->grid : (hit_blocks > 65536 ? 65536 : hit_blocks));
+  const int64_t hit_blocks = g.hit_blocks;
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
-  hipLaunchKernelGGL(k_b_raygen, dim3(work), dim3(256), 0, s, E);
+  hipLaunchKernelGGL(k_b_raygen, dim3((unsigned)g.first_blocks), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_count, dim3((unsigned)((hit_blocks + 3) / 4)), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
   hipLaunchKernelGGL(k_b_bounce, dim3((unsigned)hit_blocks), dim3(TILE), 0, s, E);
-  hipLaunchKernelGGL(k_b_final, dim3(work), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_final, dim3((unsigned)g.final_blocks), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_finish, dim3(1), dim3(1024), 0, s, E);
   return hipGetLastError() == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
 }
@@ -546,12 +552,6 @@ int32_t splat_engine_create(int64_t arena_bytes, int32_t max_slots, splat_engine
   }
   e->h_stage0[0] = 0;
   e->slots.resize(max_slots);
-  int cus = 256;
-  hipDeviceProp_t prop;
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-    cus = prop.multiProcessorCount;
-  e->grid = cus * 4;
   *out = e;
   return SPLAT_OK;
 }
@@ -651,12 +651,23 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   bool any = false;
   int64_t groups = 0;  // upper bound of the live 64-ray groups in this step
-  for (auto& h : e->slots) {
+  int64_t first = 0, last = 0;  // work items of the first-stage and last-stage kernels
+  int64_t first_rg = 0, first_clear = 0, first_keys = 0;
+  for (size_t i = 0; i < e->slots.size(); ++i) {
+    const HostSlot& h = e->slots[i];
     if (!h.used || h.stage < 0) continue;
     any = true;
     groups += (h.R + HIT_ITEM - 1) / HIT_ITEM;
+    if (h.stage == 0) {
+      if (h.depth == 0) first_rg += (h.R + TILE - 1) / TILE;
+      first_clear += (e->h_slots[i].tex_n + CLEAR_ITEM - 1) / CLEAR_ITEM;
+      first_keys += (h.R + 4 * HIT_ITEM - 1) / (4 * HIT_ITEM);
+    }
+    if (h.stage == h.depth) last += (h.R + TILE - 1) / TILE;
   }
   if (!any) return SPLAT_OK;
+  first = first_rg > first_clear ? first_rg : first_clear;
+  first = first > first_keys ? first : first_keys;
   if (!e->admits.empty()) {
     // the slot records of the frames admitted since the last step (one copy per table over the
     // range they span: the other records in it are unchanged copies of their device records),
@@ -679,16 +690,19 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   // hitter grid: a power of two of workgroups (4 waves each) that covers every item the live
   // counts can need, so a few graphs serve any mix of frames
   const int64_t need = (max_hit_items(groups) + 3) / 4;
-  int64_t hit_blocks = 1;
-  while (hit_blocks < need) hit_blocks <<= 1;
-  if (hit_blocks > (int64_t)1 << 24) return SPLAT_ERR_UNSUPPORTED;  // grid of at most 2^32 threads
+  StepGrids grids{1, 1, 1};
+  while (grids.hit_blocks < need) grids.hit_blocks <<= 1;
+  if (grids.hit_blocks > (int64_t)1 << 24) return SPLAT_ERR_UNSUPPORTED;  // grid of at most 2^32 threads
+  // the striding kernels: powers of two up to 65536 workgroups (a few graph variants in all)
+  while (grids.first_blocks < first && grids.first_blocks < 65536) grids.first_blocks <<= 1;
+  while (grids.final_blocks < last && grids.final_blocks < 65536) grids.final_blocks <<= 1;
   if (e->use_graph) {
-    hipGraphExec_t& ge = e->graphs[hit_blocks];
+    hipGraphExec_t& ge = e->graphs[grids.key()];
     if (!ge) {
       hipGraph_t g;
       if (!e->cap_stream && hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess) return SPLAT_ERR_HIP;
       if (hipStreamBeginCapture(e->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return SPLAT_ERR_HIP;
-      int32_t st = launch_step_kernels(e, hit_blocks, e->cap_stream);
+      int32_t st = launch_step_kernels(e, grids, e->cap_stream);
       if (hipStreamEndCapture(e->cap_stream, &g) != hipSuccess || st != SPLAT_OK) return SPLAT_ERR_HIP;
       hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -698,7 +712,7 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
       }
     }
     if (hipGraphLaunch(ge, s) != hipSuccess) return SPLAT_ERR_HIP;
-  } else if (int32_t st = launch_step_kernels(e, hit_blocks, s)) {
+  } else if (int32_t st = launch_step_kernels(e, grids, s)) {
     return st;
   }
   Pending p;
