@@ -84,7 +84,6 @@ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 
 // one workgroup of MAX_SLOTS threads: per-slot item counts -> exclusive prefixes
 __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
-  __shared__ int32_t buf[MAX_SLOTS];
   const int t = threadIdx.x;
   // frames admitted since the last step start at stage 0 (their slot records were uploaded
   // with the step); the list is consumed
@@ -115,37 +114,53 @@ __global__ __launch_bounds__(MAX_SLOTS) void k_plan(EngineDev E) {
   }
   // the hitter's splits: one factor for the tree walks of all frames, a flat-scan tail split for
   // frames with few live rays (both from the totals, so the items fit the host's grid)
-  auto block_sum = [&](int32_t v) {
-    buf[t] = v;
-    __syncthreads();
-    for (int off = MAX_SLOTS / 2; off > 0; off >>= 1) {
-      if (t < off) buf[t] += buf[t + off];
-      __syncthreads();
+  // (sums and prefix sums over the slots: within each wave by shuffles, then across the waves
+  // through LDS -- a few barriers in all, not a tree of them per kind)
+  constexpr int NW = MAX_SLOTS / 64;
+  __shared__ int32_t wtot[N_KINDS + 2][NW];
+  const int lane = t & 63, wv = t >> 6;
+  auto wave_incl = [&](int32_t v) {  // inclusive scan within the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t u = __shfl_up(v, off);
+      if (lane >= off) v += u;
     }
-    const int32_t r = buf[0];
-    __syncthreads();
-    return r;
+    return v;
   };
-  const int32_t g_walk = block_sum(tail ? 0 : cnt[K_HIT]);
-  const int32_t g_tail = block_sum(tail ? cnt[K_HIT] : 0);
+  const int32_t iw = wave_incl(tail ? 0 : cnt[K_HIT]), it = wave_incl(tail ? cnt[K_HIT] : 0);
+  if (lane == 63) {
+    wtot[N_KINDS][wv] = iw;
+    wtot[N_KINDS + 1][wv] = it;
+  }
+  __syncthreads();
+  int32_t g_walk = 0, g_tail = 0;
+  for (int j = 0; j < NW; ++j) {
+    g_walk += wtot[N_KINDS][j];
+    g_tail += wtot[N_KINDS + 1][j];
+  }
   const int32_t S_walk = split_factor(g_walk, MAX_SPLIT);
   const int32_t S_slot = tail ? tail_split_factor(g_tail, leaves) : S_walk;
   if (in) E.hit_split[t] = tail ? -S_slot : S_slot;
   cnt[K_WAVE] = cnt[K_HIT] * S_slot;
   cnt[K_KEYS] = s == 0 && S_slot > 1 ? cnt[K_HIT] : 0;  // without splits the hitter stores every key
+  // exclusive prefixes of every kind (slots past n_slots count nothing, so the total is theirs)
+  int32_t inc[N_KINDS];
+#pragma unroll
+  for (int k = 0; k < N_KINDS; ++k) inc[k] = wave_incl(cnt[k]);
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < N_KINDS; ++k) wtot[k][wv] = inc[k];
+  __syncthreads();
+#pragma unroll
   for (int k = 0; k < N_KINDS; ++k) {
-    buf[t] = cnt[k];
-    __syncthreads();
-    for (int off = 1; off < MAX_SLOTS; off <<= 1) {
-      int32_t v = t >= off ? buf[t - off] : 0;
-      __syncthreads();
-      buf[t] += v;
-      __syncthreads();
+    int32_t before = 0, total = 0;
+    for (int j = 0; j < NW; ++j) {
+      const int32_t v = wtot[k][j];
+      before += j < wv ? v : 0;
+      total += v;
     }
     int32_t* P = E.pre + k * (MAX_SLOTS + 1);
-    if (t < E.n_slots) P[t] = t == 0 ? 0 : buf[t - 1];
-    if (t == 0) P[E.n_slots] = buf[E.n_slots - 1];
-    __syncthreads();
+    if (t < E.n_slots) P[t] = before + inc[k] - cnt[k];
+    if (t == 0) P[E.n_slots] = total;
   }
 }
 
