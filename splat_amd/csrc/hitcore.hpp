@@ -276,8 +276,9 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       scan_records16(S.recs + rg.x + m, rg.y - m < 16 ? rg.y - m : 16, L, best, bi);
   };
 
-  // the records every ray tests before the structure (large triangles)
-  if (S.n_top > 0) {
+  // the records every ray tests before the structure (large triangles): in one split only, the
+  // splits' keys merge into the minimum over all of them
+  if (S.n_top > 0 && split == 0) {
     const int32_t nt = S.n_top;
     if constexpr (STATS) {
       n_alg += valid ? (unsigned long long)nt : 0ull;
