@@ -106,6 +106,28 @@ def test_engine_batch_equals_plain(scenes, use_graph):
     eng.close()
 
 
+def test_engine_many_slots(scenes):
+    """More slots than one wave's 64 lanes: a work item's slot lookup counts the prefixes 64 at a
+    time, so items of slots past 64 take the lookup's later passes; every frame still equals its
+    single-frame trace."""
+    specs = _frames(80, seed=7)
+    frames = [_make(s, scenes) for s in specs]
+    want = [_plain(f) for f in frames]
+    eng = api.Engine(1 << 30, max_slots=150)
+    for k, f in enumerate(frames):
+        assert _admit(eng, f, 500 + k)
+    done, steps = [], 0
+    while len(done) < len(frames):
+        eng.step()
+        steps += 1
+        done += eng.poll(wait=True)
+        assert steps <= 20
+    assert sorted(done) == [500 + k for k in range(len(frames))]
+    for f, w in zip(frames, want):
+        _check(f, w)
+    eng.close()
+
+
 def test_engine_continuous_admission_reuses_slots(scenes):
     """Frames arrive one per step while others are mid-flight; 3 slots serve 10 frames."""
     specs = _frames(10, seed=2)
