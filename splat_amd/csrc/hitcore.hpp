@@ -331,11 +331,11 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       // parent too and the leaves reached are those of a level-by-level walk, with half the chain
       // of dependent loads. Entered grandchildren go on a stack, farthest first for the wave's
       // first live ray along the split axes (nearer child, then its nearer half: measured faster
-      // than ordering by that ray's entry distances), and are popped without a re-test. The stack holds wave-uniform node
-      // indices, one per lane of a VGPR: a step pops one entry and pushes at most four, over at
-      // most 12 steps (24 levels), so it never holds more than 37 of the 64 lanes. Leaves are
-      // scanned as soon as they are reached, nearer first, each re-tested with the hits found
-      // meanwhile.
+      // than ordering by that ray's entry distances), and are popped without a re-test. The stack
+      // holds wave-uniform node indices, one per lane of a VGPR: a step pops one entry and pushes
+      // at most four, over at most 12 steps (24 levels), so it never holds more than 37 of the 64
+      // lanes. Leaves are scanned as soon as they are reached, nearer first, each re-tested with
+      // the hits found meanwhile.
       const int32_t root = n_sub + split;
       int32_t stack = 0, sp = 0;  // lane k of `stack`: entry k
       const int32_t my_lane = threadIdx.x & 63;
