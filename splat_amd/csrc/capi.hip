@@ -196,10 +196,14 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
     mark(2 * (k - 1) + 1);
     P.bounce = k;
     launch_bounce_count(cur, R, live, kc, P, gen, w.tile_counts, s);
-    launch_scan_tiles(w.tile_counts, n_tiles, live, R, w.tile_offsets, &w.counters[k], s);
+    // after the first bounce (whose tiles are all live: a scan launch costs less than every
+    // workgroup summing them all), frames of up to SCAN_LOCAL_TILES tiles let the bounce pass sum
+    // the live tiles' counts itself, one launch fewer
+    const bool local_scan = k >= 2 && n_tiles <= SCAN_LOCAL_TILES;
+    if (!local_scan) launch_scan_tiles(w.tile_counts, n_tiles, live, R, w.tile_offsets, &w.counters[k], s);
     mark_part(2 * (k - 1));
-    launch_bounce(cur, R, live, kc, bv ? kn : nullptr, P, lm, gen, w.tile_offsets, &w.counters[k], nxt, final_rays,
-                  hit_final, s);
+    launch_bounce(cur, R, live, kc, bv ? kn : nullptr, P, lm, gen, local_scan ? w.tile_counts : w.tile_offsets,
+                  local_scan, &w.counters[k], nxt, final_rays, hit_final, s);
     mark_part(2 * (k - 1) + 1);
     std::swap(cur, nxt);
     std::swap(kc, kn);

@@ -95,10 +95,14 @@ void launch_frame_start(float* tex, int64_t tex_n, int64_t* count0, int64_t R, u
 // live rays src -> dst; hit_idx from keys when keys != nullptr
 void launch_frame_end(const splat_adray_t* src, splat_adray_t* dst, int64_t R, const int64_t* d_count,
                       const uint64_t* keys, int32_t* hit_idx, hipStream_t s);
-// lm == nullptr: no splat (P.tex ignored); keys_next (nullable): miss keys at the next positions
+// lm == nullptr: no splat (P.tex ignored); keys_next (nullable): miss keys at the next positions.
+// local_scan: tile_sums are the count pass's tile counts (R within SCAN_LOCAL_TILES tiles) and the
+// bounce pass sums them itself, writing the ACTIVE total to d_total; otherwise tile_sums and
+// d_total are launch_scan_tiles' offsets and total.
+constexpr int64_t SCAN_LOCAL_TILES = 4096;
 void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
                    uint64_t* keys_next, const BounceParams& P, const splat_lightmap_params_t* lm,
-                   const splat_pinhole_t* cam, const int32_t* tile_offsets, const int64_t* d_total,
+                   const splat_pinhole_t* cam, const int32_t* tile_sums, bool local_scan, int64_t* d_total,
                    splat_adray_t* out_active, splat_adray_t* out_rest, int32_t* hit_rest, hipStream_t s);
 // hit index per ray from hit_idx, or from the packed keys when keys != nullptr
 void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const uint64_t* keys,

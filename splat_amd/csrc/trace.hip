@@ -59,16 +59,54 @@ __global__ __launch_bounds__(TILE) void k_bounce(const splat_adray_t* __restrict
                                                  const uint64_t* __restrict__ keys, uint64_t* __restrict__ keys_next,
                                                  BounceParams P, const splat_lightmap_params_t lm,
                                                  const splat_pinhole_t cam, int32_t from_cam,
-                                                 const int32_t* __restrict__ tile_offsets,
-                                                 const int64_t* __restrict__ d_total,
+                                                 const int32_t* __restrict__ tile_sums, int32_t local_scan,
+                                                 int64_t* __restrict__ d_total,
                                                  splat_adray_t* __restrict__ out_active,
                                                  splat_adray_t* __restrict__ out_rest, int32_t* __restrict__ hit_rest) {
   const int64_t A = live_count_t(R, d_count);
   const int64_t base = (int64_t)blockIdx.x * TILE;
-  if (base >= A) return;
+  if (base >= A) {
+    if (local_scan && blockIdx.x == 0 && threadIdx.x == 0) *d_total = 0;  // no live ray: none stays ACTIVE
+    return;
+  }
   const int nvalid = (int)(A - base < TILE ? A - base : TILE);
-  const int64_t off_a = tile_offsets[blockIdx.x];
-  const int64_t off_r = *d_total + (base - off_a);
+  int64_t off_a, total;
+  if (local_scan) {
+    // tile_sums are the count pass's tile counts (at most SCAN_LOCAL_TILES): the workgroup sums
+    // those before its tile and all of them itself, in place of a scan launch; workgroup 0
+    // publishes the total (the next bounce's live count)
+    const int32_t n = (int32_t)((A + TILE - 1) / TILE), tile = (int32_t)blockIdx.x;
+    int32_t before = 0, all = 0;
+    for (int32_t i = threadIdx.x; i < n; i += TILE) {
+      const int32_t c = tile_sums[i];
+      all += c;
+      before += i < tile ? c : 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      before += __shfl_xor(before, off);
+      all += __shfl_xor(all, off);
+    }
+    __shared__ int32_t red[2][TILE / 64];
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = before;
+      red[1][threadIdx.x >> 6] = all;
+    }
+    __syncthreads();
+    before = 0;
+    all = 0;
+#pragma unroll
+    for (int w = 0; w < TILE / 64; ++w) {
+      before += red[0][w];
+      all += red[1][w];
+    }
+    off_a = before;
+    total = all;
+    if (tile == 0 && threadIdx.x == 0) *d_total = total;
+  } else {  // tile_sums are the scan's tile offsets, d_total its total
+    off_a = tile_sums[blockIdx.x];
+    total = *d_total;
+  }
+  const int64_t off_r = total + (base - off_a);
   if (from_cam)
     bounce_tile<true>(rays + base, keys + base, keys_next, nvalid, P, lm, cam, base, off_a, off_r, out_active, out_rest,
                       hit_rest);
@@ -220,7 +258,7 @@ void launch_frame_end(const splat_adray_t* src, splat_adray_t* dst, int64_t R, c
 }
 void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const uint64_t* keys,
                    uint64_t* keys_next, const BounceParams& P, const splat_lightmap_params_t* lm,
-                   const splat_pinhole_t* cam, const int32_t* tile_offsets, const int64_t* d_total,
+                   const splat_pinhole_t* cam, const int32_t* tile_sums, bool local_scan, int64_t* d_total,
                    splat_adray_t* out_active, splat_adray_t* out_rest, int32_t* hit_rest, hipStream_t s) {
   if (R <= 0) return;
   splat_lightmap_params_t l;
@@ -232,7 +270,8 @@ void launch_bounce(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
   BounceParams Q = P;
   if (!lm) Q.tex = nullptr;
   hipLaunchKernelGGL(k_bounce, dim3((unsigned)((R + TILE - 1) / TILE)), dim3(TILE), 0, s, rays, R, d_count, keys,
-                     keys_next, Q, l, c, cam ? 1 : 0, tile_offsets, d_total, out_active, out_rest, hit_rest);
+                     keys_next, Q, l, c, cam ? 1 : 0, tile_sums, local_scan ? 1 : 0, d_total, out_active, out_rest,
+                     hit_rest);
 }
 
 void launch_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx, const uint64_t* keys,
