@@ -8,10 +8,9 @@
 // A step is one fixed sequence of launches for every slot at once: a planning kernel turns the
 // device-side live counts into per-kind work-item prefixes (and the hitter's split factor); the
 // hitter runs one wave per (64-ray group, split) item of any frame, so frames of any size and
-// scene cost share the chip; the bounce pass takes one tile per workgroup over the hitter's grid,
-// the other stages walk their items grid-stride. Launch shapes depend
-// only on the hitter's grid size class, so each class is captured into a HIP graph once and
-// replayed. The host tracks stages itself (every admitted frame advances exactly one stage per
+// scene cost share the chip; the bounce pass takes one tile per workgroup, the other stages walk
+// their items grid-stride. The grids are powers of two sized by the host from the frames' ray
+// counts and stages, so each set of them is captured into a HIP graph once and replayed. The host tracks stages itself (every admitted frame advances exactly one stage per
 // step), so it never reads device state to learn that a frame has finished: an event recorded
 // after each step tells when.
 #include <hip/hip_runtime.h>
@@ -242,7 +241,7 @@ __global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
 }
 
 // count pass of the fused bounce (bounce.hpp): ACTIVE rays per tile from the hit keys, one tile
-// per wave (its grid holds four waves per tile-count bound: hit_blocks >= the tile count)
+// per wave (its grid holds a wave per tile of the host's bound)
 __global__ __launch_bounds__(TILE) void k_b_count(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
@@ -312,9 +311,8 @@ __global__ __launch_bounds__(1024) void k_b_scan(EngineDev E) {
 // One tile per workgroup (no loop: a grid-stride loop over the tiles made the compiler keep the
 // slot's parameters live across iterations, 188 VGPRs and 2 waves per SIMD). Slot fields read one
 // by one and the light-map parameters in LDS bring it to 128 VGPRs (4 waves; a few spilled
-// registers cost less than the lost wave). The grid is the step's hitter grid, which covers every
-// tile: tiles <= groups / 4 + live slots and hit_blocks >= (max(groups, 32768) + 8192) / 4 with at
-// most 1024 slots.
+// registers cost less than the lost wave). The grid holds a workgroup per tile of the host's bound
+// (the ray counts of the frames that bounce in this step), which covers every live tile.
 __global__ __launch_bounds__(TILE, 4) void k_b_bounce(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_TILE * (MAX_SLOTS + 1);
@@ -489,12 +487,17 @@ struct splat_engine {
 
 namespace {
 
-// Grids of one step: hit_blocks >= the tile count, so the bounce pass takes one tile per workgroup
-// over it and the count pass one tile per wave over a quarter of it; the first-stage and last-stage
-// kernels stride over grids sized by the host for the frames at those stages (1 when none is).
+// Grids of one step, sized by the host from the frames' ray counts and stages: the hitter's covers
+// every item the live counts can need; tile_blocks >= the tile count of the frames that bounce in
+// this step, so the bounce pass takes one tile per workgroup over it and the count pass one tile
+// per wave over a quarter of it; the first-stage and last-stage kernels stride over grids for the
+// frames at those stages (1 when none is).
 struct StepGrids {
-  int64_t hit_blocks, first_blocks, final_blocks;
-  int64_t key() const { return hit_blocks | (first_blocks << 25) | (final_blocks << 42); }
+  int64_t hit_blocks, tile_blocks, first_blocks, final_blocks;  // powers of two
+  int64_t key() const {
+    auto lg = [](int64_t v) { return (int64_t)__builtin_ctzll((unsigned long long)v); };
+    return lg(hit_blocks) | (lg(tile_blocks) << 6) | (lg(first_blocks) << 12) | (lg(final_blocks) << 18);
+  }
 };
 
 int32_t launch_step_kernels(splat_engine* e, const StepGrids& g, hipStream_t s) {
@@ -503,9 +506,9 @@ int32_t launch_step_kernels(splat_engine* e, const StepGrids& g, hipStream_t s) 
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
   hipLaunchKernelGGL(k_b_raygen, dim3((unsigned)g.first_blocks), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
-  hipLaunchKernelGGL(k_b_count, dim3((unsigned)((hit_blocks + 3) / 4)), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_count, dim3((unsigned)((g.tile_blocks + 3) / 4)), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
-  hipLaunchKernelGGL(k_b_bounce, dim3((unsigned)hit_blocks), dim3(TILE), 0, s, E);
+  hipLaunchKernelGGL(k_b_bounce, dim3((unsigned)g.tile_blocks), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_final, dim3((unsigned)g.final_blocks), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_finish, dim3(1), dim3(1024), 0, s, E);
   return hipGetLastError() == hipSuccess ? SPLAT_OK : SPLAT_ERR_HIP;
@@ -667,6 +670,7 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   bool any = false;
   int64_t groups = 0;  // upper bound of the live 64-ray groups in this step
   int64_t first = 0, last = 0;  // work items of the first-stage and last-stage kernels
+  int64_t tiles = 0;            // bound of the bounce tiles: frames that bounce in this step
   int64_t first_rg = 0, first_clear = 0, first_keys = 0;
   for (size_t i = 0; i < e->slots.size(); ++i) {
     const HostSlot& h = e->slots[i];
@@ -679,6 +683,7 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
       first_keys += (h.R + 4 * HIT_ITEM - 1) / (4 * HIT_ITEM);
     }
     if (h.stage == h.depth) last += (h.R + TILE - 1) / TILE;
+    else tiles += (h.R + TILE - 1) / TILE;
   }
   if (!any) return SPLAT_OK;
   first = first_rg > first_clear ? first_rg : first_clear;
@@ -705,12 +710,15 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   // hitter grid: a power of two of workgroups (4 waves each) that covers every item the live
   // counts can need, so a few graphs serve any mix of frames
   const int64_t need = (max_hit_items(groups) + 3) / 4;
-  StepGrids grids{1, 1, 1};
+  StepGrids grids{1, 1, 1, 1};
   while (grids.hit_blocks < need) grids.hit_blocks <<= 1;
   if (grids.hit_blocks > (int64_t)1 << 24) return SPLAT_ERR_UNSUPPORTED;  // grid of at most 2^32 threads
   // the striding kernels: powers of two up to 65536 workgroups (a few graph variants in all)
   while (grids.first_blocks < first && grids.first_blocks < 65536) grids.first_blocks <<= 1;
   while (grids.final_blocks < last && grids.final_blocks < 65536) grids.final_blocks <<= 1;
+  // one bounce workgroup (and one count wave) per tile of the frames that bounce: tiles <=
+  // groups / 4 + slots, so this never passes hit_blocks
+  while (grids.tile_blocks < tiles) grids.tile_blocks <<= 1;
   if (e->use_graph) {
     hipGraphExec_t& ge = e->graphs[grids.key()];
     if (!ge) {
