@@ -502,7 +502,8 @@ def main() -> int:
         "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "roofline": {
             "kernel": "k_hit_bv (bounding-volume hitter)",
-            "bound": "valu",
+            "bound": "mfma",  # the compute-bound category; the peak is the FP32 dense figure
+            "pipe": "fp32 VALU (per-ray scalar FP32 box and triangle tests; no matrix instructions)",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
