@@ -508,6 +508,8 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            # the FP32 peak counts packed FMA; these kernels are built without packed FP32 ops
+            "frac_of_unpacked_peak": round(achieved / (FP32_PEAK_TFLOPS / 2), 4),
             "traffic": traffic,
             "traffic_source": traffic_src,
             "flop_per_launch": flop_per_launch,
