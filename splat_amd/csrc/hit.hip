@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
 // BV hitter over (64-ray group, split) work items, one wave each. The split factor follows the
 // live count on the device, so bounces with few, incoherent survivors still spread over the whole
 // chip; the splits of a group merge their (t, index) keys with a 64-bit atomic minimum. The grid
-// covers the most items any live count can need (max_hit_items); waves past the count leave.
+// covers the most items any live count can need (max_hit_items_one); waves past the count leave.
 template <bool STATS, int MODE>
 __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
                                                 const int64_t* __restrict__ d_count, BVScene scene,
@@ -350,7 +350,7 @@ void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
                    uint64_t* stats, hipStream_t s, const splat_pinhole_t* cam) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
-  dim3 grid((unsigned)((max_hit_items((R + 63) / 64) + 3) / 4)), block(256);
+  dim3 grid((unsigned)((max_hit_items_one((R + 63) / 64) + 3) / 4)), block(256);
   splat_pinhole_t c;
   if (cam) c = *cam;
   else std::memset(&c, 0, sizeof(c));
