@@ -65,15 +65,17 @@ def test_device_build_equals_host_build(name, depth):
     assert _bytes(host, a.d_groups + 32, (nodes - 1) * 32) == _bytes(dev, b.d_groups + 32, (nodes - 1) * 32)
 
 
-@pytest.mark.parametrize("name,depth", [("c1", 9), ("small_c2", 4), ("menger2", 12)])
-def test_device_built_hits_equal_brute_force(name, depth):
-    """Deep builds leave empty leaves (c1 at depth 9: 320 triangles over 512 leaves)."""
+@pytest.mark.parametrize("name,depth,n_pts", [("c1", 9, 3000), ("small_c2", 4, 3000), ("menger2", 12, 3000),
+                                              ("c1", 9, 6000), ("menger2", 12, 6000), ("menger2", 15, 6000)])
+def test_device_built_hits_equal_brute_force(name, depth, n_pts):
+    """Deep builds leave empty leaves (c1 at depth 9: 320 triangles over 512 leaves). 3000 rays
+    (47 groups of 64) take the flat scan of few live rays, 6000 the tree walk."""
     import torch
 
     sc = _scenes()[name]
     tris_d = api.to_device(sc.ftris)
     dev = api.HIPBVHitter.build_on_device(tris_d, depth)
-    pts = H.sample_surface_points(sc.ftris, 3000, seed=depth)
+    pts = H.sample_surface_points(sc.ftris, n_pts, seed=depth)
     rays_np = H.rays_towards(pts, seed=depth + 1)
     rays = api.to_device(rays_np)
     tr = api.HIPTracer(rays)

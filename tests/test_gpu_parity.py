@@ -194,6 +194,10 @@ def test_next_hit_bv_equals_bf(build, depth, group_size):
     (3, 0, 0, None), (5, 0, 1, None), (5, 0, 2, None), (4, 4, 0, None), (4, -1, 0, None), (1, 0, 0, None),
     # at most 64 live 64-ray groups: the tree's leaves are scanned flat, unless walk has bit 4
     (5, 0, 0, 3000), (8, 0, 0, 4096), (1, 0, 0, 100), (5, 0, 4, 3000), (5, 0, 5, 3000), (5, 0, 6, 3000),
+    # the wave walk takes two levels per step: even and odd depths below the splits' roots, four
+    # or two children at the last step, deep trees with split roots (walk bit 4: few rays, 32 splits)
+    (2, 0, 0, None), (4, 0, 0, None), (6, 0, 0, None), (7, 0, 0, None), (10, 0, 0, None),
+    (6, 0, 4, 3000), (9, 0, 4, 3000), (10, 0, 4, 3000),
 ])
 def test_next_hit_bv_top_list_equals_bf(depth, group_size, walk, n_rays):
     """Large triangles tested directly before the structure (tree_partition), every structure
