@@ -459,3 +459,25 @@ def test_trace_frame_vs_oracle():
     fr.trace()  # a second frame on the same buffers gives the same map (tex is cleared per frame)
     _sync()
     assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
+
+
+def test_frame_where_every_ray_finishes_at_the_first_bounce():
+    """Only diffuse triangles: after the first bounce no ray stays ACTIVE, so the later bounces
+    run with no live ray (the bounce pass, which sums the tile counts itself after the first
+    bounce, then publishes the zero live count); the frame still equals the oracle's."""
+    sc = scene.assemble([], [scene.ground_quad(-1.5, 3.0)], "ground only")
+    cam = scene.camera_c2(seed=8, height=64, width=96, jitter=0.025)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=8)
+    lm = abi.lightmap_params(64, 64)
+    want_r, want_h, _ = O.run_evolution(O.raygen(cam), sc.ftris, p)
+    assert (want_h > 1).any() and (want_h == 1).any()  # some rays on the ground, some missing it
+    want_tex, want_r2 = O.light_map(want_r, want_h, sc.ftris, lm)
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem = bv.object_partition(sc.ftris, 1)
+    for hitter in (api.HIPBVHitter(tris_d, bvs, mem), api.HIPBruteForceHitter(api.to_device(sc.ntris()))):
+        fr = api.Frame(cam, hitter, tris_d, p, lm)
+        fr.trace()
+        _sync()
+        assert (fr.hit_idx.cpu().numpy() == want_h).all()
+        assert api.from_device(fr.rays, abi.ADRAY).tobytes() == want_r2.tobytes()
+        assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
