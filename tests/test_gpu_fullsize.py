@@ -83,6 +83,17 @@ def test_two_icospheres_1mp_sampled_vs_oracle():
     assert fr.tex.sum().item() > 0
 
 
+def test_two_icospheres_past_the_local_tile_sum_sampled_vs_oracle():
+    """1032x1024 rays: 4128 tiles, past the 4096 up to which the bounce pass sums the live tiles'
+    counts itself after the first bounce, so every bounce takes the separate scan; same checks."""
+    sc = scene.scene_c2(seed=2)
+    cam = scene.camera_c2(seed=5, height=1032, width=1024, jitter=0.025)
+    fr, p = _frame(sc, cam, 4, abi.lightmap_params(1024, 1024), seed=77)
+    assert fr.R == 1032 * 1024 and (fr.R + 255) // 256 > 4096
+    _check_sample(sc, cam, fr, p, n=2000, seed=3)
+    assert fr.tex.sum().item() > 0
+
+
 def test_large_scene_invariants_and_sample():
     """~178k triangles (Menger sponge + subdivision-6 icosphere), 6 bounces, BV hitter; the full
     2048^2 x 8 wavelengths frame is 33.5M rays: invariants on the whole frame; per-ray parity on
