@@ -135,27 +135,6 @@ int32_t check_optics(const splat_optics_t* o) {
   return SPLAT_OK;
 }
 
-int32_t check_lightmap(const splat_lightmap_params_t* lm) {
-  if (!lm || lm->width < 1 || lm->height < 1 || lm->channels < 1) return SPLAT_ERR_INVALID_ARG;
-  if (lm->n_lambda < 0 || lm->n_lambda > SPLAT_MAX_LAMBDA) return SPLAT_ERR_UNSUPPORTED;
-  if (lm->n_delta < 1 || lm->n_delta > SPLAT_MAX_DELTA) return SPLAT_ERR_UNSUPPORTED;
-  switch (lm->weight_mode) {
-    case SPLAT_WEIGHTS_IDENTITY:
-      if (lm->channels != (lm->n_lambda > 0 ? lm->n_lambda : 1)) return SPLAT_ERR_INVALID_ARG;
-      break;
-    case SPLAT_WEIGHTS_TABLE:
-      if (lm->channels > 3 || lm->n_lambda < 1) return SPLAT_ERR_INVALID_ARG;
-      break;
-    case SPLAT_WEIGHTS_CURVES:
-      if (lm->channels > 3) return SPLAT_ERR_INVALID_ARG;
-      for (int c = 0; c < lm->channels; ++c)
-        if (lm->curve_n[c] < 2 || lm->curve_n[c] > SPLAT_MAX_CURVE) return SPLAT_ERR_INVALID_ARG;
-      break;
-    default:
-      return SPLAT_ERR_INVALID_ARG;
-  }
-  return SPLAT_OK;
-}
 
 // the device-resident bounce loop; results land in `final_rays`. The live rays start in w.s1, or,
 // with cam != nullptr (and depth >= 1), are that pinhole's rays, generated where the first
@@ -235,6 +214,32 @@ int32_t check_run(const splat_adray_t* rays, int64_t R, const splat_hitset_t* hs
 }
 
 }  // namespace
+
+// (shared with the engine's admission)
+int32_t spl::check_lightmap(const splat_lightmap_params_t* lm) {
+  if (!lm || lm->width < 1 || lm->height < 1 || lm->channels < 1) return SPLAT_ERR_INVALID_ARG;
+  if (lm->n_lambda < 0 || lm->n_lambda > SPLAT_MAX_LAMBDA) return SPLAT_ERR_UNSUPPORTED;
+  if (lm->n_delta < 1 || lm->n_delta > SPLAT_MAX_DELTA) return SPLAT_ERR_UNSUPPORTED;
+  switch (lm->weight_mode) {
+    case SPLAT_WEIGHTS_IDENTITY:
+      if (lm->channels != (lm->n_lambda > 0 ? lm->n_lambda : 1)) return SPLAT_ERR_INVALID_ARG;
+      break;
+    case SPLAT_WEIGHTS_TABLE:
+      if (lm->channels > 3 || lm->n_lambda < 1) return SPLAT_ERR_INVALID_ARG;
+      break;
+    case SPLAT_WEIGHTS_CURVES:
+      if (lm->channels > 3) return SPLAT_ERR_INVALID_ARG;
+      for (int c = 0; c < lm->channels; ++c) {
+        if (lm->curve_n[c] < 2 || lm->curve_n[c] > SPLAT_MAX_CURVE) return SPLAT_ERR_INVALID_ARG;
+        for (int j = 0; j + 1 < lm->curve_n[c]; ++j)  // the abscissae ascend (not NaN, not decreasing)
+          if (!(lm->curve_x[c][j] <= lm->curve_x[c][j + 1])) return SPLAT_ERR_INVALID_ARG;
+      }
+      break;
+    default:
+      return SPLAT_ERR_INVALID_ARG;
+  }
+  return SPLAT_OK;
+}
 
 extern "C" {
 

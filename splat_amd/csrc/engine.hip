@@ -602,9 +602,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   if (rq->hs.kind != SPLAT_HITTER_BV || !rq->hs.d_recs || !rq->hs.d_boxes || !rq->hs.d_ranges) return SPLAT_ERR_INVALID_ARG;
   if (rq->depth < 0 || rq->depth > MAXD) return SPLAT_ERR_UNSUPPORTED;
   const splat_lightmap_params_t& lm = rq->lm;
-  if (lm.width < 1 || lm.height < 1 || lm.channels < 1 || lm.n_delta < 1 || lm.n_delta > SPLAT_MAX_DELTA ||
-      lm.n_lambda < 0 || lm.n_lambda > SPLAT_MAX_LAMBDA || lm.weight_mode < 0 || lm.weight_mode > 2)
-    return SPLAT_ERR_INVALID_ARG;
+  if (int32_t err = check_lightmap(&lm)) return err;
   if (rq->optics.ior_model != SPLAT_IOR_REFERENCE_LINEAR && rq->optics.ior_model != SPLAT_IOR_CAUCHY) return SPLAT_ERR_INVALID_ARG;
   const int64_t R = (int64_t)cam.height * cam.width * cam.samples_per_pixel;
   if (R > (int64_t)INT32_MAX / 2) return SPLAT_ERR_UNSUPPORTED;

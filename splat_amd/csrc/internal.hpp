@@ -26,6 +26,8 @@ void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, f
 // centres most, bit 2 = the right child lies lower on it, bit 3 = empty subtree.
 constexpr int32_t TREE_AXIS = 3, TREE_SWAP = 4, TREE_EMPTY = 8;
 constexpr int32_t MAX_TREE_LEVELS = 24;
+// light-map parameters a kernel can use (sizes in range; curve abscissae ascending)
+int32_t check_lightmap(const splat_lightmap_params_t* lm);
 // the device BV build (bvbuild.hip): member list and ranges of 1 + 2^depth BVs (the large
 // triangles, then the object-median leaves) of triangles 2..n+1; h_summary <- {n_large, n - n_large}
 // once the stream reaches it

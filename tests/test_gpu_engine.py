@@ -188,3 +188,25 @@ def test_engine_frame_matches_oracle(scenes):
     assert api.from_device(f["rays"], abi.ADRAY).tobytes() == rays.tobytes()
     assert tex.sum() > 0 and H.rel_l2(f["tex"].cpu().numpy(), tex) < 1e-4
     eng.close()
+
+
+def test_engine_admission_checks_the_light_map(scenes):
+    """Admission applies the C ABI's light-map check: curve point counts within SPLAT_MAX_CURVE
+    and ascending abscissae (the kernels read the curves by those counts)."""
+    from splat_amd import lib as SL
+
+    f = _make(dict(si=1, h=32, w=32, depth=2, fr=1.0, seed=3, mode="rgb", nd=1, lmw=40, lmh=40), scenes)
+    eng = api.Engine(1 << 28, max_slots=2)
+    lm_bad_n = abi.lightmap_params(40, 40)
+    lm_bad_n.curve_n[0] = abi.MAX_CURVE + 1
+    lm_bad_x = abi.lightmap_params(40, 40)
+    lm_bad_x.curve_x[1][3], lm_bad_x.curve_x[1][4] = lm_bad_x.curve_x[1][4], lm_bad_x.curve_x[1][3]
+    for lm in (lm_bad_n, lm_bad_x):
+        with pytest.raises(SL.SplatError) as err:
+            eng.admit(cam=f["cam"], hitter=f["hitter"], tris=f["tris"], params=f["p"], lm=lm, tex=f["tex"], tag=1)
+        assert err.value.code == -1  # SPLAT_ERR_INVALID_ARG
+    assert _admit(eng, f, 2)  # a valid request still goes in
+    while eng.in_flight:
+        eng.step()
+        eng.poll(wait=True)
+    eng.close()

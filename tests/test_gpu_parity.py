@@ -481,3 +481,21 @@ def test_frame_where_every_ray_finishes_at_the_first_bounce():
         assert (fr.hit_idx.cpu().numpy() == want_h).all()
         assert api.from_device(fr.rays, abi.ADRAY).tobytes() == want_r2.tobytes()
         assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
+
+
+def test_light_map_curves_must_ascend():
+    """The retina curves' abscissae must ascend: a frame with two swapped breakpoints is refused
+    before anything runs (SPLAT_ERR_INVALID_ARG)."""
+    from splat_amd import lib as SL
+
+    sc = scene.scene_c1()
+    tris_d = api.to_device(sc.ftris)
+    hitter = api.HIPBruteForceHitter(api.to_device(sc.ntris()))
+    cam = scene.camera_c1(seed=1, height=16, width=16)
+    p = abi.run_params(2, sc.first_diffuse, math.nan, seed=1)
+    lm = abi.lightmap_params(32, 32)
+    lm.curve_x[2][5], lm.curve_x[2][6] = lm.curve_x[2][6], lm.curve_x[2][5]
+    fr = api.Frame(cam, hitter, tris_d, p, lm)
+    with pytest.raises(SL.SplatError) as err:
+        fr.trace()
+    assert err.value.code == -1
