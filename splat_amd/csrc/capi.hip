@@ -77,15 +77,6 @@ PartWs carve_part(void* ws, int64_t R) {
   return w;
 }
 
-int32_t check_hitset(const splat_hitset_t* hs) {
-  if (!hs || !hs->d_recs || hs->n_tris < 1) return SPLAT_ERR_INVALID_ARG;
-  if (hs->kind == SPLAT_HITTER_BV) {
-    if (!hs->d_boxes || !hs->d_ranges || hs->n_bv < 0) return SPLAT_ERR_INVALID_ARG;
-  } else if (hs->kind != SPLAT_HITTER_BF) {
-    return SPLAT_ERR_INVALID_ARG;
-  }
-  return SPLAT_OK;
-}
 
 int bf_waves(const splat_hitset_t* hs, int64_t R) {
   return hs->tune_waves > 0 ? hs->tune_waves : choose_bf_waves(R, hs->n_tris);
@@ -214,6 +205,21 @@ int32_t check_run(const splat_adray_t* rays, int64_t R, const splat_hitset_t* hs
 }
 
 }  // namespace
+
+// (shared with the engine's admission) the prepared hit set's sizes within what the kernels
+// assume: the tree walk's stack holds the pending nodes of at most MAX_TREE_LEVELS levels
+int32_t spl::check_hitset(const splat_hitset_t* hs) {
+  if (!hs || !hs->d_recs || hs->n_tris < 1) return SPLAT_ERR_INVALID_ARG;
+  if (hs->kind == SPLAT_HITTER_BV) {
+    if (!hs->d_boxes || !hs->d_ranges || hs->n_bv < 0) return SPLAT_ERR_INVALID_ARG;
+    if (hs->tree_levels < 0 || hs->tree_levels > MAX_TREE_LEVELS + 1 || hs->top_bvs < 0 || hs->top_recs < 0 ||
+        hs->group_size < 0)
+      return SPLAT_ERR_INVALID_ARG;
+  } else if (hs->kind != SPLAT_HITTER_BF) {
+    return SPLAT_ERR_INVALID_ARG;
+  }
+  return SPLAT_OK;
+}
 
 // (shared with the engine's admission)
 int32_t spl::check_lightmap(const splat_lightmap_params_t* lm) {

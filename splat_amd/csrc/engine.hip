@@ -599,7 +599,8 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   if (!e || !rq || !rq->tex || !rq->tris || rq->n_tris < 1) return SPLAT_ERR_INVALID_ARG;
   const splat_pinhole_t& cam = rq->cam;
   if (cam.height < 1 || cam.width < 1 || cam.samples_per_pixel < 1 || !(cam.fov_half_sin > 0.f)) return SPLAT_ERR_INVALID_ARG;
-  if (rq->hs.kind != SPLAT_HITTER_BV || !rq->hs.d_recs || !rq->hs.d_boxes || !rq->hs.d_ranges) return SPLAT_ERR_INVALID_ARG;
+  if (rq->hs.kind != SPLAT_HITTER_BV) return SPLAT_ERR_INVALID_ARG;
+  if (int32_t err = check_hitset(&rq->hs)) return err;
   if (rq->depth < 0 || rq->depth > MAXD) return SPLAT_ERR_UNSUPPORTED;
   const splat_lightmap_params_t& lm = rq->lm;
   if (int32_t err = check_lightmap(&lm)) return err;

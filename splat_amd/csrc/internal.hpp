@@ -28,6 +28,8 @@ constexpr int32_t TREE_AXIS = 3, TREE_SWAP = 4, TREE_EMPTY = 8;
 constexpr int32_t MAX_TREE_LEVELS = 24;
 // light-map parameters a kernel can use (sizes in range; curve abscissae ascending)
 int32_t check_lightmap(const splat_lightmap_params_t* lm);
+// a prepared hit set the kernels can use (pointers present, tree depth within MAX_TREE_LEVELS)
+int32_t check_hitset(const splat_hitset_t* hs);
 // the device BV build (bvbuild.hip): member list and ranges of 1 + 2^depth BVs (the large
 // triangles, then the object-median leaves) of triangles 2..n+1; h_summary <- {n_large, n - n_large}
 // once the stream reaches it

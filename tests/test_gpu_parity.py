@@ -499,3 +499,30 @@ def test_light_map_curves_must_ascend():
     with pytest.raises(SL.SplatError) as err:
         fr.trace()
     assert err.value.code == -1
+
+
+def test_hit_set_deeper_than_the_walk_supports_is_refused():
+    """The tree walk keeps the pending nodes of at most 24 levels: a hit set that claims a deeper
+    tree is refused before anything runs, by next_hit and by engine admission."""
+    from splat_amd import lib as SL
+
+    sc = H.small_c2(2)
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem, n_top = bv.tree_partition(sc.ftris, 5)
+    hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=0, n_top=n_top)
+    hitter.hs.tree_levels = 26
+    rays = api.to_device(O.raygen(scene.camera_c2(seed=4, height=16, width=16)))
+    tr = api.HIPTracer(rays)
+    with pytest.raises(SL.SplatError) as err:
+        api.next_hit_(tr, hitter, rays)
+    assert err.value.code == -1
+    import torch
+
+    cam = scene.camera_c2(seed=4, height=16, width=16)
+    lm = abi.lightmap_params(16, 16)
+    eng = api.Engine(1 << 26, max_slots=1)
+    with pytest.raises(SL.SplatError) as err:
+        eng.admit(cam=cam, hitter=hitter, tris=tris_d, params=abi.run_params(2, sc.first_diffuse, math.nan, seed=1),
+                  lm=lm, tex=torch.zeros((lm.channels, 16, 16), dtype=torch.float32, device="cuda"), tag=1)
+    assert err.value.code == -1
+    eng.close()
