@@ -295,6 +295,18 @@ int32_t splat_next_hit_bf(const splat_adray_t* rays, int64_t R, const splat_ntri
                           int32_t n_tris_len, int32_t* hit_idx, void* ws, size_t ws_bytes,
                           void* stream);
 
+/* splat_next_hit without a workspace. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same result as splat_next_hit; a BV hit set's per-ray keys live in a
+ * library-owned device array of SPLAT_NOALLOC_MAX_RAYS keys (8 B each, 16 MiB), one copy per
+ * device, held until the process ends. R above SPLAT_NOALLOC_MAX_RAYS returns
+ * SPLAT_ERR_CAPACITY. Every call on one device shares that device's array (each call
+ * re-initialises the keys it uses), so calls that overlap on one device -- on different streams
+ * or host threads -- race: serialise them. Allocates nothing (no hipMalloc/hipFree). */
+#define SPLAT_NOALLOC_MAX_RAYS (1 << 21)
+int32_t splat_next_hit_noalloc(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
+                               const splat_hitset_t* hs, int32_t* hit_idx, void* stream);
+
 /* rays .= evolve_ray.(rays, hit_idx, tris[hit_idx], rndm, first_diffuse)
  * (src/tracer.jl:181-201, :118-179). Random number per ray: force_rand when not NaN
  * (src/tracer.jl:256-258), else rndm[r] when rndm != NULL, else f(seed, bounce, ray.dest). */

@@ -13,6 +13,9 @@ import numpy as np
 
 # ---- status / enums (include/splat_hip.h) ------------------------------------------------
 SPLAT_OK = 0
+SPLAT_ERR_INVALID_ARG, SPLAT_ERR_WORKSPACE, SPLAT_ERR_HIP, SPLAT_ERR_UNSUPPORTED = -1, -2, -3, -4
+SPLAT_ERR_CAPACITY, SPLAT_ERR_BUDGET = -5, -6
+NOALLOC_MAX_RAYS = 1 << 21  # splat_next_hit_noalloc's scratch capacity (rays)
 RAY_DIFFUSE, RAY_ACTIVE, RAY_INFINITY = 0, 1, 2
 IOR_REFERENCE_LINEAR, IOR_CAUCHY = 0, 1
 COMPAT_DY_AT_X, COMPAT_REFRACT_NORMAL_LAMBDA = 1, 2

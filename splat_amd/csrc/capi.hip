@@ -2,6 +2,7 @@
 // the device-resident bounce loop.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -468,6 +469,46 @@ int32_t splat_next_hit_bf(const splat_adray_t* rays, int64_t R, const splat_ntri
   splat_hitset_t hs;
   if (int32_t e = splat_hitset_prepare_bf(n_tris, N, ws, ws_bytes, &hs, stream)) return e;
   return splat_next_hit(rays, R, nullptr, &hs, hit_idx, nullptr, 0, stream);
+}
+
+// ---- allocation-free next_hit: the BV hitter's key scratch in a library-owned device array ----
+// Every call on one device shares that device's copy of `scratch` (the packed key of each ray,
+// re-initialised by every call before the hitter reads it), so calls that overlap on one device
+// race. Its address is taken once per device, with that device current; host code never uses
+// the array's name itself, which on the host is a stand-in and not a device address.
+namespace {
+static __device__ __align__(256) uint64_t scratch[SPLAT_NOALLOC_MAX_RAYS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+int32_t device_scratch(uint64_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return SPLAT_ERR_HIP;
+  if (dev >= MAX_DEVICES) return SPLAT_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SPLAT_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SPLAT_OK;
+}
+}  // namespace
+
+int32_t splat_next_hit_noalloc(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const splat_hitset_t* hs,
+                               int32_t* hit_idx, void* stream) {
+  if (R < 0 || (R > 0 && (!rays || !hit_idx))) return SPLAT_ERR_INVALID_ARG;
+  if (R > SPLAT_NOALLOC_MAX_RAYS) return SPLAT_ERR_CAPACITY;
+  if (int32_t e = check_hitset(hs)) return e;
+  uint64_t* keys = nullptr;
+  if (hs->kind == SPLAT_HITTER_BV) {
+    if (int32_t e = device_scratch(&keys)) return e;
+  }
+  hit_index(rays, R, d_count, hs, hit_idx, keys, S(stream));
+  return launch_status();
 }
 
 int32_t splat_evolve(splat_adray_t* rays, int64_t R, const int64_t* d_count, const int32_t* hit_idx,

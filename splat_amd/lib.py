@@ -54,6 +54,7 @@ SIGNATURES = {
     "splat_next_hit_ws_bytes": (sz, [i64]),
     "splat_next_hit": (i32, [P, i64, P, P, P, P, sz, P]),
     "splat_next_hit_bf": (i32, [P, i64, P, i32, P, P, sz, P]),
+    "splat_next_hit_noalloc": (i32, [P, i64, P, P, P, P]),
     "splat_evolve": (i32, [P, i64, P, P, P, i32, P, f32, u64, i32, i32, P, P]),
     "splat_partition_ws_bytes": (sz, [i64]),
     "splat_partition_active": (i32, [P, P, i64, P, P, P, sz, P]),

@@ -68,6 +68,9 @@ def test_argument_errors_without_a_gpu():
     hs = abi.HitSet()
     assert lib.splat_next_hit(None, 0, None, L.ref(hs), None, None, 0, None) == -1  # no records
     assert lib.splat_next_hit_ws_bytes(1000) >= 8 * 1000 and lib.splat_next_hit_ws_bytes(-1) == 0
+    assert lib.splat_next_hit_noalloc(None, -1, None, None, None, None) == abi.SPLAT_ERR_INVALID_ARG
+    assert lib.splat_next_hit_noalloc(C.c_void_p(8), abi.NOALLOC_MAX_RAYS + 1, None, L.ref(hs), C.c_void_p(8),
+                                      None) == abi.SPLAT_ERR_CAPACITY
     p = abi.run_params(65, 2)
     assert lib.splat_run_evolution(C.c_void_p(8), 10, L.ref(hs), C.c_void_p(8), 3, C.c_void_p(8), L.ref(p),
                                    C.c_void_p(8), 1 << 30, None, None) == -1
