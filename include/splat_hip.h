@@ -159,7 +159,7 @@ typedef struct splat_lightmap_params {
   float delta[SPLAT_MAX_DELTA];
   float weights[SPLAT_MAX_LAMBDA * 3]; /* SPLAT_WEIGHTS_TABLE: [k][c] */
   int32_t curve_n[3];                  /* SPLAT_WEIGHTS_CURVES: points per channel (>= 2) */
-  float curve_x[3][SPLAT_MAX_CURVE];   /* ascending abscissae (nm; else SPLAT_ERR_INVALID_ARG) */
+  float curve_x[3][SPLAT_MAX_CURVE];   /* strictly ascending abscissae (nm; else SPLAT_ERR_INVALID_ARG) */
   float curve_y[3][SPLAT_MAX_CURVE];   /* ordinates (already normalised)                  */
 } splat_lightmap_params_t;
 

@@ -492,6 +492,10 @@ class Engine:
             self._keep.pop(t, None)
         return tags
 
+    def synchronize(self) -> None:
+        """Wait for all device work (the engine's steps run on the caller's streams)."""
+        _torch().cuda.synchronize()
+
     def sync_steps(self, keep: int) -> None:
         """Block until at most `keep` launched steps are still running."""
         L.check(self._lib.splat_engine_sync_steps(self.h, int(keep)), "engine_sync_steps")

@@ -214,8 +214,17 @@ int32_t spl::check_hitset(const splat_hitset_t* hs) {
   if (hs->kind == SPLAT_HITTER_BV) {
     if (!hs->d_boxes || !hs->d_ranges || hs->n_bv < 0) return SPLAT_ERR_INVALID_ARG;
     if (hs->tree_levels < 0 || hs->tree_levels > MAX_TREE_LEVELS + 1 || hs->top_bvs < 0 || hs->top_recs < 0 ||
-        hs->group_size < 0)
+        hs->group_size < 0 || hs->n_groups < 0)
       return SPLAT_ERR_INVALID_ARG;
+    // the structure must reach every BV below the top list (the kernels visit nothing else)
+    const int32_t top = hs->top_bvs > 0 && hs->top_bvs < hs->n_bv ? hs->top_bvs : 0;
+    const int64_t below = (int64_t)hs->n_bv - top;
+    if (top > 0 && hs->top_recs > hs->total_members) return SPLAT_ERR_INVALID_ARG;
+    if (hs->d_groups && hs->tree_levels > 0) {
+      if (((int64_t)1 << (hs->tree_levels - 1)) < below) return SPLAT_ERR_INVALID_ARG;
+    } else if (hs->d_groups && hs->group_size > 0 && hs->n_groups > 0) {
+      if ((int64_t)hs->n_groups * hs->group_size < below) return SPLAT_ERR_INVALID_ARG;
+    }
   } else if (hs->kind != SPLAT_HITTER_BF) {
     return SPLAT_ERR_INVALID_ARG;
   }
@@ -238,8 +247,9 @@ int32_t spl::check_lightmap(const splat_lightmap_params_t* lm) {
       if (lm->channels > 3) return SPLAT_ERR_INVALID_ARG;
       for (int c = 0; c < lm->channels; ++c) {
         if (lm->curve_n[c] < 2 || lm->curve_n[c] > SPLAT_MAX_CURVE) return SPLAT_ERR_INVALID_ARG;
-        for (int j = 0; j + 1 < lm->curve_n[c]; ++j)  // the abscissae ascend (not NaN, not decreasing)
-          if (!(lm->curve_x[c][j] <= lm->curve_x[c][j + 1])) return SPLAT_ERR_INVALID_ARG;
+        // the abscissae ascend strictly (not NaN, no repeat: a zero-width segment divides by zero)
+        for (int j = 0; j + 1 < lm->curve_n[c]; ++j)
+          if (!(lm->curve_x[c][j] < lm->curve_x[c][j + 1])) return SPLAT_ERR_INVALID_ARG;
       }
       break;
     default:

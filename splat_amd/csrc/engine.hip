@@ -667,7 +667,8 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   if (!e) return SPLAT_ERR_INVALID_ARG;
   hipStream_t s = static_cast<hipStream_t>(stream);
   bool any = false;
-  int64_t groups = 0;  // upper bound of the live 64-ray groups in this step
+  int64_t groups = 0;       // upper bound of the live 64-ray groups in this step
+  int64_t tail_groups = 0;  // upper bound of those in frames that may scan flat (<= TAIL_GROUPS each)
   int64_t first = 0, last = 0;  // work items of the first-stage and last-stage kernels
   int64_t tiles = 0;            // bound of the bounce tiles: frames that bounce in this step
   int64_t first_rg = 0, first_clear = 0, first_keys = 0;
@@ -675,7 +676,9 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
     const HostSlot& h = e->slots[i];
     if (!h.used || h.stage < 0) continue;
     any = true;
-    groups += (h.R + HIT_ITEM - 1) / HIT_ITEM;
+    const int64_t g = (h.R + HIT_ITEM - 1) / HIT_ITEM;
+    groups += g;
+    tail_groups += g < TAIL_GROUPS ? g : TAIL_GROUPS;
     if (h.stage == 0) {
       if (h.depth == 0) first_rg += (h.R + TILE - 1) / TILE;
       first_clear += (e->h_slots[i].tex_n + CLEAR_ITEM - 1) / CLEAR_ITEM;
@@ -708,7 +711,7 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   }
   // hitter grid: a power of two of workgroups (4 waves each) that covers every item the live
   // counts can need, so a few graphs serve any mix of frames
-  const int64_t need = (max_hit_items(groups) + 3) / 4;
+  const int64_t need = (max_hit_items(groups, tail_groups) + 3) / 4;
   StepGrids grids{1, 1, 1, 1};
   while (grids.hit_blocks < need) grids.hit_blocks <<= 1;
   if (grids.hit_blocks > (int64_t)1 << 24) return SPLAT_ERR_UNSUPPORTED;  // grid of at most 2^32 threads

@@ -482,11 +482,15 @@ __device__ __forceinline__ int32_t tail_split_factor(int64_t groups64, int32_t l
   while (S < TAIL_MAX_SPLIT && S * 16 <= leaves && groups64 * (S * 2) <= TAIL_ITEMS) S *= 2;
   return S;
 }
-// Work items (waves) the hitter needs for at most `groups64` live groups: the split factor only
-// grows while groups64 * S < SPLIT_TARGET_ITEMS, so it never takes the count past twice that;
-// flat-scan tails add at most TAIL_ITEMS (a step of many frames can hold both kinds).
-inline __host__ __device__ int64_t max_hit_items(int64_t groups64) {
-  return (groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS) + TAIL_ITEMS;
+// Work items (waves) the hitter needs for at most `groups64` live groups, of which at most
+// `tail_groups64` can belong to flat-scan tails (a frame scans flat only with at most TAIL_GROUPS
+// live groups): the walk split only grows while groups * S < SPLIT_TARGET_ITEMS, so the walks
+// never take more than max(groups, 2 SPLIT_TARGET_ITEMS); the tail split only grows while
+// tail groups * S <= TAIL_ITEMS and is 1 above that, so the tails take at most
+// max(tail groups, TAIL_ITEMS). A step of many frames can hold both kinds.
+inline __host__ __device__ int64_t max_hit_items(int64_t groups64, int64_t tail_groups64) {
+  return (groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS) +
+         (tail_groups64 > TAIL_ITEMS ? tail_groups64 : TAIL_ITEMS);
 }
 // The same for a launch over one frame, whose items are all tree walks or all a flat-scan tail:
 // the larger bound, not the sum (every wave past the items still has to be dispatched).

@@ -81,7 +81,7 @@ def run_requests(engine: api.Engine, scenes: list[ResidentScene], requests: list
             rq.tex = torch.empty((rq.lm.channels, rq.lm.height, rq.lm.width), dtype=torch.float32, device="cuda")
     by_tag = {rq.tag: rq for rq in requests}
     queue = list(requests)
-    torch.cuda.synchronize()
+    engine.synchronize()
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
     steps = refused = max_in_flight = 0
@@ -93,6 +93,11 @@ def run_requests(engine: api.Engine, scenes: list[ResidentScene], requests: list
             sc = scenes[rq.scene]
             if not engine.admit(cam=rq.cam, hitter=sc.hitter, tris=sc.tris, params=rq.params, lm=rq.lm, tex=rq.tex,
                                 tag=rq.tag):
+                if not engine.in_flight:
+                    # an empty engine refused it: the frame is larger than the whole arena, and
+                    # waiting would never admit it
+                    raise ValueError(f"request {rq.tag} ({rq.rays} rays) needs {engine.frame_bytes(rq.cam)} bytes of "
+                                     f"engine arena; the empty engine has {engine.arena_free}")
                 refused += 1
                 break
             rq.admitted = t

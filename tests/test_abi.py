@@ -108,3 +108,51 @@ def test_retina_curves_normalised():
     d = abi.upres(4)
     assert np.allclose(d, [-0.375, -0.125, 0.125, 0.375])
     assert abi.upres(1)[0] == 0.0
+
+
+def _fake_bv_hitset(n_bv: int, **kw) -> abi.HitSet:
+    hs = abi.HitSet()
+    hs.kind, hs.n_tris, hs.n_bv, hs.total_members, hs.max_members = abi.HITTER_BV, 40, n_bv, 60, 6
+    hs.d_recs = hs.d_boxes = hs.d_ranges = hs.d_groups = 256  # never dereferenced: refused or sized out first
+    for k, v in kw.items():
+        setattr(hs, k, v)
+    return hs
+
+
+def test_hit_set_structure_must_cover_its_bvs_without_a_gpu():
+    """A BV hit set whose tree or group level cannot reach every BV below its top list (the
+    kernels visit only those), or whose top list claims more records than the set holds, is
+    refused (SPLAT_ERR_INVALID_ARG); a covering one passes the check and stops at the workspace
+    size (SPLAT_ERR_WORKSPACE), before anything is launched."""
+    lib = L.lib()
+    p = abi.run_params(2, 3)
+
+    def code(hs):
+        return lib.splat_run_evolution(C.c_void_p(8), 10, L.ref(hs), C.c_void_p(8), 40, C.c_void_p(8), L.ref(p),
+                                       C.c_void_p(8), 0, None, None)
+
+    # tree over 2^(levels-1) leaves: 8 BVs below a top list of 2 need 8 leaves (levels 4)
+    assert code(_fake_bv_hitset(10, tree_levels=4, top_bvs=2, top_recs=5)) == abi.SPLAT_ERR_WORKSPACE
+    assert code(_fake_bv_hitset(10, tree_levels=3, top_bvs=2, top_recs=5)) == abi.SPLAT_ERR_INVALID_ARG
+    assert code(_fake_bv_hitset(10, tree_levels=4, top_bvs=0)) == abi.SPLAT_ERR_INVALID_ARG  # 10 > 8 leaves
+    # groups: n_groups * group_size must cover the BVs below the top list
+    assert code(_fake_bv_hitset(10, group_size=4, n_groups=2, top_bvs=2, top_recs=5)) == abi.SPLAT_ERR_WORKSPACE
+    assert code(_fake_bv_hitset(10, group_size=4, n_groups=1, top_bvs=2, top_recs=5)) == abi.SPLAT_ERR_INVALID_ARG
+    # the top list's records lie inside the set's records
+    assert code(_fake_bv_hitset(10, tree_levels=4, top_bvs=2, top_recs=61)) == abi.SPLAT_ERR_INVALID_ARG
+    assert code(_fake_bv_hitset(10, tree_levels=4, top_bvs=2, top_recs=60)) == abi.SPLAT_ERR_WORKSPACE
+    # flat (no structure over the BVs): nothing to cover
+    assert code(_fake_bv_hitset(12)) == abi.SPLAT_ERR_WORKSPACE
+
+
+def test_repeated_curve_breakpoints_are_refused_without_a_gpu():
+    """A repeated first or last breakpoint would give a zero-width end segment, whose slope divides
+    by zero for every wavelength outside the table: refused (SPLAT_ERR_INVALID_ARG)."""
+    lib = L.lib()
+    for c, j in ((0, 0), (1, -1), (2, 4)):
+        lm = abi.lightmap_params(8, 8)
+        n = lm.curve_n[c]
+        j = j % (n - 1)
+        lm.curve_x[c][j + 1] = lm.curve_x[c][j]
+        assert lib.splat_light_map(C.c_void_p(8), 1, None, C.c_void_p(8), C.c_void_p(8), 3, L.ref(lm), C.c_void_p(8),
+                                   None) == abi.SPLAT_ERR_INVALID_ARG

@@ -128,6 +128,33 @@ def test_engine_many_slots(scenes):
     eng.close()
 
 
+def test_engine_many_flat_tails_beside_a_split_walk(scenes):
+    """188 frames of 64 ray groups each (each small enough to scan the leaves flat) in one step with
+    a 700-group frame whose tree walk is split 32 ways: 22,400 walk items plus 12,032 flat items,
+    more than the hitter grid held before the flat-item bound counted the tail frames' groups.
+    Every frame still equals its single-frame trace."""
+    rng = np.random.default_rng(11)
+    specs = [dict(si=1, h=224, w=200, depth=3, fr=math.nan, seed=901, mode="rgb", nd=1, lmw=96, lmh=80)]
+    for k in range(188):
+        specs.append(dict(si=int(rng.integers(0, 3)), h=64, w=64, depth=int(rng.integers(1, 3)), fr=math.nan,
+                          seed=int(rng.integers(1, 1 << 30)), mode="mono", nd=1, lmw=48, lmh=40))
+    frames = [_make(s, scenes) for s in specs]
+    want = [_plain(f) for f in frames]
+    eng = api.Engine(1 << 30, max_slots=256)
+    for k, f in enumerate(frames):
+        assert _admit(eng, f, 700 + k)
+    done, steps = [], 0
+    while len(done) < len(frames):
+        eng.step()
+        steps += 1
+        done += eng.poll(wait=True)
+        assert steps <= 10
+    assert sorted(done) == [700 + k for k in range(len(frames))]
+    for f, w in zip(frames, want):
+        _check(f, w)
+    eng.close()
+
+
 def test_engine_continuous_admission_reuses_slots(scenes):
     """Frames arrive one per step while others are mid-flight; 3 slots serve 10 frames."""
     specs = _frames(10, seed=2)
