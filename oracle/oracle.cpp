@@ -591,10 +591,12 @@ template <class T> inline T curve_eval(const splat_lightmap_params_t& lm, int c,
   return FMA(slope, lam - x1, y1);
 }
 
-// atomic_spectrum_kernel2 (src/ray_imagers.jl:78-113) for one ray, sequential accumulation:
-// splat (k, iy, ix) in the reference's thread order (CartesianIndices((dx, dy, lambda))).
-template <class T, class A>
-void splat_one(const RayT<T>& r, const FTriT<T>& F, const splat_lightmap_params_t& lm, A* tex) {
+// atomic_spectrum_kernel2 (src/ray_imagers.jl:78-113) for one ray: every splat sample (k, iy, ix)
+// in the reference's thread order (CartesianIndices((dx, dy, lambda))), handed to
+// fn(k, lam, U*W, V*H, I) with the texture coordinates scaled to texels (before ceil and clamp)
+// and the cosine intensity; samples with a non-finite (u, v) are skipped, as there.
+template <class T, class Fn>
+void splat_samples(const RayT<T>& r, const FTriT<T>& F, const splat_lightmap_params_t& lm, Fn&& fn) {
   if (r.status != SPLAT_RAY_DIFFUSE) return;
   const int nl = lm.n_lambda > 0 ? lm.n_lambda : 1;
   const T W = T((float)lm.width), H = T((float)lm.height);
@@ -617,32 +619,50 @@ void splat_one(const RayT<T>& r, const FTriT<T>& F, const splat_lightmap_params_
         T U = FMA(F.v[9].x, w, FMA(F.v[8].x, bv, F.v[7].x * bu));
         T V = FMA(F.v[9].y, w, FMA(F.v[8].y, bv, F.v[7].y * bu));
         if (!std::isfinite(U) || !std::isfinite(V)) continue;
-        T fi = std::ceil(U * W), fj = std::ceil(V * H);
-        fi = fi < T(1) ? T(1) : (fi > W ? W : fi);
-        fj = fj < T(1) ? T(1) : (fj > H ? H : fj);
-        int64_t ii = (int64_t)fi - 1, jj = (int64_t)fj - 1;
         // cosine_shading (src/geo.jl:283-286)
         V3<T> nrm = optical_normal(F, p);
         T I = std::fabs(dot3(nrm, normalize3(d)));
-        int64_t plane = (int64_t)lm.width * lm.height;
-        int64_t base = ii + (int64_t)lm.width * jj;
-        if (lm.weight_mode == SPLAT_WEIGHTS_IDENTITY) {
-#pragma omp atomic
-          tex[base + plane * k] += (A)I;
-        } else if (lm.weight_mode == SPLAT_WEIGHTS_TABLE) {
-          for (int c = 0; c < lm.channels; ++c) {
-#pragma omp atomic
-            tex[base + plane * c] += (A)(I * T(lm.weights[k * 3 + c]));
-          }
-        } else {
-          for (int c = 0; c < lm.channels; ++c) {
-#pragma omp atomic
-            tex[base + plane * c] += (A)(I * curve_eval(lm, c, lam));
-          }
-        }
+        fn(k, lam, U * W, V * H, I);
       }
     }
   }
+}
+
+// the splats of one ray accumulated into tex: texel (clamp(ceil(U W), 1, W), clamp(ceil(V H), 1, H))
+template <class T, class A>
+void splat_one(const RayT<T>& r, const FTriT<T>& F, const splat_lightmap_params_t& lm, A* tex) {
+  const T W = T((float)lm.width), H = T((float)lm.height);
+  const int64_t plane = (int64_t)lm.width * lm.height;
+  splat_samples(r, F, lm, [&](int k, T lam, T uw, T vh, T I) {
+    T fi = std::ceil(uw), fj = std::ceil(vh);
+    fi = fi < T(1) ? T(1) : (fi > W ? W : fi);
+    fj = fj < T(1) ? T(1) : (fj > H ? H : fj);
+    int64_t ii = (int64_t)fi - 1, jj = (int64_t)fj - 1;
+    int64_t base = ii + (int64_t)lm.width * jj;
+    if (lm.weight_mode == SPLAT_WEIGHTS_IDENTITY) {
+#pragma omp atomic
+      tex[base + plane * k] += (A)I;
+    } else if (lm.weight_mode == SPLAT_WEIGHTS_TABLE) {
+      for (int c = 0; c < lm.channels; ++c) {
+#pragma omp atomic
+        tex[base + plane * c] += (A)(I * T(lm.weights[k * 3 + c]));
+      }
+    } else {
+      for (int c = 0; c < lm.channels; ++c) {
+#pragma omp atomic
+        tex[base + plane * c] += (A)(I * curve_eval(lm, c, lam));
+      }
+    }
+  });
+}
+
+// distance (in texels) from a scaled texture coordinate to the nearest texel edge that changes
+// its clamped bin: the integers 1 .. n-1
+template <class T> inline T edge_distance(T x, int32_t n) {
+  if (n < 2) return std::numeric_limits<T>::infinity();
+  T k = std::round(x);
+  k = k < T(1) ? T(1) : (k > T(n - 1) ? T(n - 1) : k);
+  return std::fabs(x - k);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1059,8 +1079,43 @@ void oracle_light_map_f64(const splat_adray_t* rays, int64_t R, const int32_t* h
   }
 }
 
+// Binning margins of each ray's splats in FP64: texel_margin[r] = the smallest distance (texels)
+// from any of its samples' scaled texture coordinates to a bin edge; edge_margin[r] = the smallest
+// barycentric coordinate of its final hit point on its diffuse triangle (negative outside; +inf
+// for rays that are not DIFFUSE, or whose final hit misses). A sample within rounding of a texel
+// edge (or a hit within rounding of a triangle edge) lands in a precision-dependent bin.
+void oracle_splat_margins_f64(const splat_adray_t* rays, int64_t R, const int32_t* hit_idx, const splat_ftri_t* tris,
+                              int32_t N, const splat_lightmap_params_t* lm, double* texel_margin, double* edge_margin) {
+  std::vector<FTriT<double>> F = load_ftris<double>(tris, N);
+  const double inf = std::numeric_limits<double>::infinity();
+  for (int64_t r = 0; r < R; ++r) {
+    texel_margin[r] = inf;
+    edge_margin[r] = inf;
+    RayT<double> ray = load_ray<double>(rays[r]);
+    if (ray.status != SPLAT_RAY_DIFFUSE) continue;
+    int32_t i = hit_idx[r];
+    RayT<double> fe = final_evolution_one<double>(ray, i, F[i - 1]);
+    if (fe.ignore_tri == i) {
+      double bu, bv;
+      reverse_uv(F[i - 1], fe.pos, &bu, &bv);
+      edge_margin[r] = std::min(std::min(bu, bv), (1.0 - bu) - bv);
+    }
+    double m = inf;
+    splat_samples(fe, F[i - 1], *lm, [&](int, double, double uw, double vh, double) {
+      m = std::min(m, std::min(edge_distance(uw, lm->width), edge_distance(vh, lm->height)));
+    });
+    texel_margin[r] = m;
+  }
+}
+
 void oracle_raygen(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R) {
   for (int64_t q = 0; q < R; ++q) raygen_one(*cam, q, &rays[q]);
+}
+
+// the rays at the given generation indices of a pinhole frame (a sample of a frame too large to
+// generate whole on the host)
+void oracle_raygen_at(const splat_pinhole_t* cam, const int64_t* q, int64_t n, splat_adray_t* rays) {
+  for (int64_t k = 0; k < n; ++k) raygen_one(*cam, q[k], &rays[k]);
 }
 
 // rays_from_lights(lights, upscale) (src/ray_generators.jl:64-113): each light's rays in turn;

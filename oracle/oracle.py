@@ -68,6 +68,8 @@ def lib():
             "oracle_light_map_f32": (None, [P, i64, P, P, i32, P, P]),
             "oracle_light_map_f64": (None, [P, i64, P, P, i32, P, P]),
             "oracle_raygen": (None, [P, P, i64]),
+            "oracle_raygen_at": (None, [P, P, i64, P]),
+            "oracle_splat_margins_f64": (None, [P, i64, P, P, i32, P, P, P]),
             "oracle_raygen_lights": (i64, [P, i32, P, P]),
             "oracle_run_evolution_f32": (i64, [P, i64, P, i32, P, P, i32, P, P, i64, i32]),
             "oracle_trace_frame_f32": (i64, [P, P, i64, P, i32, P, P, P, P, i32, P, P, i64, i32]),
@@ -181,10 +183,30 @@ def light_map(rays, hit_idx, ftris, lm, precision="f32"):
     return tex, None
 
 
+def splat_margins(rays, hit_idx, ftris, lm):
+    """FP64 binning margins of each ray's splats: (texel_margin, edge_margin) arrays, see
+    oracle_splat_margins_f64."""
+    rays = np.ascontiguousarray(rays)
+    hit_idx = np.ascontiguousarray(hit_idx, dtype=np.int32)
+    tm = np.empty(rays.shape[0], dtype=np.float64)
+    em = np.empty(rays.shape[0], dtype=np.float64)
+    lib().oracle_splat_margins_f64(_p(rays), rays.shape[0], _p(hit_idx), _p(ftris), ftris.shape[0], _ref(lm), _p(tm),
+                                   _p(em))
+    return tm, em
+
+
 def raygen(cam):
     R = cam.height * cam.width * cam.samples_per_pixel
     rays = np.zeros(R, dtype=abi.ADRAY)
     lib().oracle_raygen(_ref(cam), _p(rays), R)
+    return rays
+
+
+def raygen_at(cam, q):
+    """The pinhole frame's rays at generation indices q (0-based), in that order."""
+    q = np.ascontiguousarray(q, dtype=np.int64)
+    rays = np.zeros(q.shape[0], dtype=abi.ADRAY)
+    lib().oracle_raygen_at(_ref(cam), _p(q), q.shape[0], _p(rays))
     return rays
 
 

@@ -42,6 +42,8 @@ class Request:
     lm: abi.LightMapParams
     arrival: float = 0.0  # seconds after the start
     tex: object = None
+    rays_out: object = None  # optional device (R, 136) uint8: the frame's final rays
+    hit_out: object = None   # optional device int32[R]: their hit indices
     admitted: float = math.nan
     done: float = math.nan
 
@@ -92,7 +94,7 @@ def run_requests(engine: api.Engine, scenes: list[ResidentScene], requests: list
             rq = queue[0]
             sc = scenes[rq.scene]
             if not engine.admit(cam=rq.cam, hitter=sc.hitter, tris=sc.tris, params=rq.params, lm=rq.lm, tex=rq.tex,
-                                tag=rq.tag):
+                                tag=rq.tag, rays_out=rq.rays_out, hit_out=rq.hit_out):
                 if not engine.in_flight:
                     # an empty engine refused it: the frame is larger than the whole arena, and
                     # waiting would never admit it

@@ -86,3 +86,53 @@ def canonical_nans(rays: np.ndarray) -> np.ndarray:
 def same_rays(a: np.ndarray, b: np.ndarray) -> bool:
     """Bit-exact equality of ray records, any NaN equal to any NaN (canonical_nans)."""
     return a.shape == b.shape and canonical_nans(a).tobytes() == canonical_nans(b).tobytes()
+
+
+def fp64_check(got32: np.ndarray, rays: np.ndarray, hit: np.ndarray, sc, force_rand: float, *, eps: float = 1e-7,
+               draws: int = 6, seed: int = 1):
+    """Per-ray comparison of one FP32 bounce (`got32`, ADRay records) with the FP64 oracle on the
+    same inputs, and each ray's conditioning.
+
+    kappa: the largest relative change of any output vector per unit relative change of the inputs,
+    over `draws` random input perturbations of relative size <= eps (vector fields, lambda, x, y)
+    through the FP64 oracle; a ray is well conditioned when kappa <= 30 (FP32 rounding, ~6e-8 per
+    operation, then moves it well under 1e-5). Rays whose discrete outcome (status, in_medium)
+    differs between FP32, FP64 or any perturbation are not compared. Vector norms are floored at
+    1e-3 (tangents near zero carry no relative precision).
+
+    Returns (err, kappa, comparable) arrays over the rays."""
+    from oracle import oracle as O
+
+    r64 = O.to64(rays)
+    w0 = O.evolve64(r64, hit, sc.ftris, force_rand=force_rand, first_diffuse=sc.first_diffuse)
+    same = (got32["status"] == w0["status"]) & (got32["in_medium"] == w0["in_medium"]) & \
+           (got32["status"] == abi.RAY_ACTIVE)
+    err = np.zeros(rays.shape[0])
+    for f in abi.VEC_FIELDS:
+        nb = np.maximum(np.linalg.norm(w0[f], axis=1), 1e-3)
+        err = np.maximum(err, np.linalg.norm(got32[f].astype(np.float64) - w0[f], axis=1) / nb)
+    rng = np.random.default_rng(seed)
+    kappa = np.zeros(rays.shape[0])
+    for _ in range(draws):
+        pert = r64.copy()
+        for f in abi.VEC_FIELDS + ["lam", "x", "y"]:
+            pert[f] = r64[f] * (1 + eps * rng.uniform(-1.0, 1.0, r64[f].shape))
+        wp = O.evolve64(pert, hit, sc.ftris, force_rand=force_rand, first_diffuse=sc.first_diffuse)
+        same &= (wp["status"] == w0["status"]) & (wp["in_medium"] == w0["in_medium"])
+        for f in abi.VEC_FIELDS:
+            nb = np.maximum(np.linalg.norm(w0[f], axis=1), 1e-3)
+            kappa = np.maximum(kappa, np.linalg.norm(wp[f] - w0[f], axis=1) / nb / eps)
+    return err, kappa, same
+
+
+def unambiguous_rays(rays: np.ndarray, hit: np.ndarray, ftris: np.ndarray, lm, *, texel_tol: float = 2e-4,
+                     edge_tol: float = 1e-6) -> np.ndarray:
+    """Mask of the rays whose light-map bins do not depend on the precision: every splat sample at
+    least `texel_tol` texels from a bin edge and the final hit point at least `edge_tol` (barycentric)
+    inside its triangle, by the FP64 oracle. Binning (ceil of u*w) is discontinuous, so a sample
+    within rounding of an edge lands in a neighbouring texel in FP32 and FP64 alike at random; the
+    FP64 comparison of a light map is over the other rays."""
+    from oracle import oracle as O
+
+    tm, em = O.splat_margins(rays, hit, ftris, lm)
+    return (tm > texel_tol) & (np.abs(em) > edge_tol)

@@ -281,20 +281,31 @@ def test_evolve_rndm_array_and_inactive_untouched(c1):
     assert got[::3].tobytes() == r[::3].tobytes()
 
 
-def test_evolve_matches_fp64(c1):
-    """Per-bounce FP32 (GPU) vs FP64 oracle on the same inputs: t via the new position, and all
-    tangents, within 1e-5 relative on rays whose discrete choices agree and are well conditioned."""
-    sc, _, rays = c1
-    hit, _ = O.next_hit(rays, sc.ntris())
-    got = _evolve_gpu(rays, hit, sc.ftris, first_diffuse=sc.first_diffuse, force_rand=1.0)
-    want = O.evolve(rays, hit, sc.ftris, force_rand=1.0, first_diffuse=sc.first_diffuse, precision="f64")
-    same = (got["status"] == want["status"]) & (got["in_medium"] == want["in_medium"]) & (got["status"] == abi.RAY_ACTIVE)
-    assert same.sum() > 1000
-    for f in abi.VEC_FIELDS:
-        a = got[f][same].astype(np.float64)
-        b = want[f][same]
-        err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-3)
-        assert np.quantile(err, 0.99) < 1e-5, (f, np.quantile(err, 0.99))
+@pytest.mark.parametrize("which", ["c1", "small_c2"])
+@pytest.mark.parametrize("force_rand", [1.0, 0.0])
+def test_evolve_matches_fp64_on_every_well_conditioned_ray(which, force_rand):
+    """Bounces 1-3 on the GPU (FP32) vs the FP64 oracle on the same inputs: every comparable ray
+    whose FP64 conditioning is kappa <= 30 (tests/helpers.py fp64_check) has position, direction
+    and all six tangents within 1e-5 relative; at least 85% of a bounce's comparable rays qualify."""
+    if which == "c1":
+        sc, cam = scene.scene_c1(), scene.camera_c1(seed=11, height=80, width=80)
+    else:
+        sc, cam = H.small_c2(3), scene.camera_c2(seed=4, height=80, width=80)
+    rays = O.raygen(cam)
+    checked = 0
+    for bounce in range(1, 4):
+        hit, _ = O.next_hit(rays, sc.ntris())
+        got = _evolve_gpu(rays, hit, sc.ftris, first_diffuse=sc.first_diffuse, force_rand=force_rand)
+        err, kappa, comparable = H.fp64_check(got, rays, hit, sc, force_rand)
+        good = comparable & (kappa <= 30)
+        assert (err[good] <= 1e-5).all(), (bounce, err[good].max())
+        if comparable.sum() >= 500:
+            assert good.sum() >= 0.85 * comparable.sum(), (bounce, good.sum(), comparable.sum())
+        checked += int(good.sum())
+        rays = got[got["status"] == abi.RAY_ACTIVE]
+        if rays.shape[0] < 50:
+            break
+    assert checked > 1000
 
 
 # ---- partition ------------------------------------------------------------------------------
@@ -346,6 +357,18 @@ def test_light_map_vs_oracle(c1, mode, spectrum, n_delta):
     got = tex.cpu().numpy()
     assert want_tex.sum() > 0
     assert H.rel_l2(got, want_tex) < 1e-4
+    # the tolerance spec: the FP64 oracle's map (FP64 final evolution, expansion and accumulation)
+    # over the rays whose splats all lie clear of a bin edge (H.unambiguous_rays)
+    keep = H.unambiguous_rays(r, hit, sc.ftris, lm)
+    assert keep.mean() > 0.98
+    rays_k = api.to_device(r[keep])
+    tr_k = api.HIPTracer(rays_k, N=n_delta)
+    tr_k.hit_idx.copy_(torch.from_numpy(np.ascontiguousarray(hit[keep])))
+    tex_k = torch.zeros_like(tex)
+    api.continuum_light_map2_(tracer=tr_k, tris=api.to_device(sc.ftris), rays=rays_k, tex=tex_k, lm=lm)
+    _sync()
+    tex64, _ = O.light_map(r[keep], hit[keep], sc.ftris, lm, precision="f64")
+    assert H.rel_l2(tex_k.cpu().numpy(), tex64) < 1e-4
 
 
 # ---- the fused bounce loop and the whole frame ---------------------------------------------
