@@ -87,6 +87,18 @@ def hitter_traffic():
     return float(t["hbm_bytes_per_launch"]), t.get("source")
 
 
+def hitter_pmc_flops():
+    """FP32 FLOP per hitter launch on this workload from the committed rocprofv3 --pmc summary:
+    SQ_INSTS_VALU_FLOPS_FP32 per dispatch times the factor the same counter reads for the
+    calibration kernel of known FLOP count (bench.py --flop-probe); None when absent."""
+    path = os.path.join(ROOT, "profiles", "k_hit_bv_c2_counters.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return float(t["valu_flops_fp32_per_dispatch"]) * float(t["flop_per_count"]), t.get("source")
+
+
 def live_counts(cam, hitter, tris, params) -> list[int]:
     """Live (ACTIVE) rays before each bounce of the frame, A_0 = R, ..., A_depth, from the same
     hitter / evolve / partition operators stepped one bounce at a time (identical rays and
@@ -132,7 +144,8 @@ def small_frame(budget_s: float = 3.0):
     tris = api.to_device(sc.ftris)
     p = abi.run_params(2, sc.first_diffuse, math.nan, seed=2024)
     lm = abi.lightmap_params(512, 512)
-    fr = api.Frame(cam, api.HIPBruteForceHitter(tris), tris, p, lm)
+    hitter = api.HIPBruteForceHitter(tris)
+    fr = api.Frame(cam, hitter, tris, p, lm)
     R = fr.R
     for _ in range(10):
         fr.trace()
@@ -143,6 +156,34 @@ def small_frame(budget_s: float = 3.0):
         fr.trace()
     torch.cuda.synchronize()
     gpu = R * K / (time.perf_counter() - t) / 1e6
+    # the brute-force hitter's launches timed with events on the frame's stream, and its
+    # algorithmic tests (every live ray x every triangle) from a counting frame
+    hip = Hip()
+    D, M = p.depth, 50
+    per = 4 * D + 2
+    hev = hip.events(per * M)
+    for m in range(M):
+        p.hit_events = C.addressof(hev) + m * per * C.sizeof(C.c_void_p)
+        p.n_hit_events = per
+        fr.trace()
+    torch.cuda.synchronize()
+    launch_ms = [hip.elapsed_ms(hev[m * per + 2 * j], hev[m * per + 2 * j + 1]) for m in range(M) for j in range(D + 1)]
+    p.hit_events, p.n_hit_events = None, 0
+    hip.destroy(hev)
+    stats = torch.zeros(5, dtype=torch.int64, device="cuda")
+    hitter.hs.d_stats = stats.data_ptr()
+    fr.trace()
+    torch.cuda.synchronize()
+    hitter.hs.d_stats = None
+    _, _, alg, exe, _ = (int(x) for x in stats.cpu().tolist())
+    avg_s = float(np.mean(launch_ms)) / 1e3
+    flop = FLOP_PER_TRI_TEST * alg / (D + 1)
+    bf = {"kernel": "k_hit_bf (brute-force hitter)", "bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+          "achieved": round(flop / avg_s / 1e12, 3), "frac": round(flop / avg_s / 1e12 / FP32_PEAK_TFLOPS, 4),
+          "frac_of_unpacked_peak": round(flop / avg_s / 1e12 / (FP32_PEAK_TFLOPS / 2), 4),
+          "flop_per_launch": flop, "flop_per_launch_executed": FLOP_PER_TRI_TEST * exe / (D + 1),
+          "avg_launch_ms": round(avg_s * 1e3, 4), "launches_per_frame": D + 1,
+          "tri_tests_per_frame": alg}
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     n, reps = 0, 0
     t = time.perf_counter()
@@ -152,7 +193,7 @@ def small_frame(budget_s: float = 3.0):
     cpu = n / (time.perf_counter() - t) / 1e6
     return {"workload": "320-triangle icosphere + plane, 256x256 rays, brute force, 2 bounces, 512x512x3 map",
             "gpu_mrays_per_s": round(gpu, 2), "cpu_mrays_per_s": round(cpu, 3), "cpu_threads": threads,
-            "cpu_frames": reps, "gpu_ms_per_frame": round(R / gpu / 1e3, 4)}
+            "cpu_frames": reps, "gpu_ms_per_frame": round(R / gpu / 1e3, 4), "roofline_bf_hitter": bf}
 
 
 def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
@@ -187,7 +228,7 @@ def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
 
 def batched_vs_plain(reps: int = 3):
     """64 mixed frames (256^2-1024^2 rays, icosphere subdiv 2-5 + plane, 4 bounces): all in one
-    engine batch (shared launches, persistent work-queue hitter, graph-replayed steps) versus the
+    engine batch (every frame's work items in shared launches, one graph replay per step) versus the
     same frames traced one frame per launch sequence."""
     import torch
 
@@ -223,27 +264,110 @@ def batched_vs_plain(reps: int = 3):
             "plain_s": round(tp, 4), "batched_s": round(tb, 4), "steps_per_batch": st.steps}
 
 
-def served(n_requests: int = 120, rate: float = 40.0):
-    """Poisson arrivals over 8 resident scenes (320 to ~178k triangles), resolution uniform over
-    256^2-2048^2, 1-16 stratified wavelengths per pixel, 4 bounces, admitted against an HBM budget
-    and continuously batched into shared steps."""
+def _sweep_point(st, reqs, rate, extra=None):
     import numpy as np
 
+    out = st.summary()
+    offered = sum(rq.rays for rq in reqs) / reqs[-1].arrival / 1e6
+    out.update({"arrival_rate_per_s": rate, "offered_mrays_per_s": round(offered, 1),
+                "achieved_vs_offered": round(out["mrays_per_s"] / offered, 3)})
+    small = np.array([(rq.done - rq.arrival) * 1e3 for rq in reqs if rq.rays <= (1 << 20)])
+    if extra == "small" and small.size:
+        out["p99_ms_requests_upto_1M_rays"] = round(float(np.percentile(small, 99)), 3)
+    return out
+
+
+def served(n_requests: int = 120, rates=(40.0, 150.0, 300.0, 600.0, 1200.0)):
+    """Poisson arrivals over 8 resident scenes (320 to ~178k triangles), resolution uniform over
+    256^2-2048^2, 1-16 stratified wavelengths per pixel, 4 bounces, admitted against an HBM budget
+    and continuously batched into shared steps; one run per arrival rate, from below to past
+    saturation (achieved below offered)."""
     from splat_amd import api, serve
 
     scenes = serve.serving_scenes()
-    reqs = serve.serving_requests(n_requests, rate, len(scenes), [s.sc.first_diffuse for s in scenes])
+    fd = [s.sc.first_diffuse for s in scenes]
     eng = api.Engine(96 << 30, max_slots=64)
-    serve.run_requests(eng, scenes, reqs[:4], realtime=False)  # warm-up: graphs, first-touch
-    reqs = serve.serving_requests(n_requests, rate, len(scenes), [s.sc.first_diffuse for s in scenes])
-    st = serve.run_requests(eng, scenes, reqs, realtime=True)
+    serve.run_requests(eng, scenes, serve.serving_requests(4, rates[0], len(scenes), fd), realtime=False)  # warm-up
+    sweep = []
+    for rate in rates:
+        reqs = serve.serving_requests(n_requests, rate, len(scenes), fd)
+        st = serve.run_requests(eng, scenes, reqs, realtime=True)
+        sweep.append(_sweep_point(st, reqs, rate, "small"))
     eng.close()
-    out = st.summary()
-    small = np.array([(rq.done - rq.arrival) * 1e3 for rq in reqs if rq.rays <= (1 << 20)])
-    out.update({"arrival_rate_per_s": rate, "hbm_budget_gib": 96,
-                "offered_mrays_per_s": round(sum(rq.rays for rq in reqs) / reqs[-1].arrival / 1e6, 1),
-                "p99_ms_requests_upto_1M_rays": round(float(np.percentile(small, 99)), 3) if small.size else None,
-                "triangles": [s.sc.n_tris for s in scenes]})
+    out = dict(sweep[0])
+    out.update({"hbm_budget_gib": 96, "triangles": [s.sc.n_tris for s in scenes], "sweep": sweep})
+    return out
+
+
+def served_1mp(n_requests: int = 300, rates=(400.0, 1500.0, 2500.0, 3500.0, 5000.0)):
+    """The latency case: Poisson arrivals of 1 MP requests (1024^2 rays, one seeded wavelength per
+    ray, 4 bounces, 1024^2 RGB light map) over the 10,240-triangle two-icosphere scene, each with
+    its own pose and light, continuously batched; p50/p99 = completion - arrival. One run per
+    arrival rate, from below to past saturation; `p99_over_33ms_from_rate` is the lowest rate
+    whose p99 exceeds 33 ms (None: none of them)."""
+    import math
+
+    from splat_amd import abi, api, scene, serve
+
+    sc = scene.scene_c2(seed=2)
+    rs = serve.ResidentScene.build(sc)
+
+    def make(rate):
+        arr = serve.poisson_arrivals(n_requests, rate, seed=7)
+        reqs = []
+        for k in range(n_requests):
+            cam = serve.lights_cam(100 + k, 1024, 1, abi.LAMBDA_UNIFORM)
+            p = abi.run_params(4, sc.first_diffuse, math.nan, seed=1000 + k)
+            reqs.append(serve.Request(k, 0, cam, p, abi.lightmap_params(1024, 1024), arrival=float(arr[k])))
+        return reqs
+
+    eng = api.Engine(16 << 30, max_slots=64)
+    serve.run_requests(eng, [rs], make(rates[0])[:8], realtime=False)  # warm-up
+    sweep = []
+    for rate in rates:
+        reqs = make(rate)
+        st = serve.run_requests(eng, [rs], reqs, realtime=True)
+        sweep.append(_sweep_point(st, reqs, rate))
+    eng.close()
+    out = dict(sweep[0])
+    over = [pt["arrival_rate_per_s"] for pt in sweep if pt["p99_ms"] > 33.0]
+    out.update({"rays_per_request": 1 << 20, "triangles": sc.n_tris, "sweep": sweep,
+                "p99_over_33ms_from_rate": over[0] if over else None})
+    return out
+
+
+def large_frame(frames: int = 3):
+    """The large-scene workload: a level-3 Menger sponge (96,000 triangles) and a subdivision-6
+    icosphere (81,920) over a plane, 2048^2 pixels x 8 stratified wavelengths (33.5M rays), BV
+    tree built on the device, 6 bounces, 2048^2 RGB light map; whole frames timed with events."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from splat_amd import abi, api, scene
+
+    sc = scene.scene_c3()
+    cam = scene.camera_c3(seed=3)
+    tris = api.to_device(sc.ftris)
+    hitter = api.HIPBVHitter.build_on_device(tris)
+    p = abi.run_params(6, sc.first_diffuse, math.nan, seed=2024)
+    fr = api.Frame(cam, hitter, tris, p, abi.lightmap_params(2048, 2048))
+    fr.trace()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(frames + 1)]
+    evs[0].record()
+    for i in range(frames):
+        fr.trace()
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    ms = [evs[i].elapsed_time(evs[i + 1]) for i in range(frames)]
+    out = {"workload": "Menger sponge level 3 + icosphere subdiv 6 + plane (177,923 triangles), 2048^2 x 8 lambda "
+                       "(33.5M rays), BV tree over 2^%d leaves built on the device, 6 bounces, 2048^2x3 map" % hitter.depth,
+           "frames": frames, "ms_per_frame": [round(x, 3) for x in ms],
+           "mrays_per_s": round(fr.R / (float(np.mean(ms)) / 1e3) / 1e6, 1)}
+    del fr, hitter, tris
+    torch.cuda.empty_cache()
     return out
 
 
@@ -314,37 +438,6 @@ def render_frame(frames: int = 20):
             "mrays_per_s": round(2 * (1 << 20) / float(np.mean(ms)) / 1e3, 1)}
 
 
-def served_1mp(n_requests: int = 300, rate: float = 400.0):
-    """The latency case: Poisson arrivals of 1 MP requests (1024^2 rays, one seeded wavelength per
-    ray, 4 bounces, 1024^2 RGB light map) over the 10,240-triangle two-icosphere scene, each with
-    its own pose and light, continuously batched; p50/p99 = completion - arrival."""
-    import math
-
-    from splat_amd import abi, api, scene, serve
-
-    sc = scene.scene_c2(seed=2)
-    rs = serve.ResidentScene.build(sc)
-    arr = serve.poisson_arrivals(n_requests, rate, seed=7)
-
-    def make():
-        reqs = []
-        for k in range(n_requests):
-            cam = serve.lights_cam(100 + k, 1024, 1, abi.LAMBDA_UNIFORM)
-            p = abi.run_params(4, sc.first_diffuse, math.nan, seed=1000 + k)
-            reqs.append(serve.Request(k, 0, cam, p, abi.lightmap_params(1024, 1024), arrival=float(arr[k])))
-        return reqs
-
-    eng = api.Engine(16 << 30, max_slots=64)
-    serve.run_requests(eng, [rs], make()[:8], realtime=False)  # warm-up
-    reqs = make()
-    st = serve.run_requests(eng, [rs], reqs, realtime=True)
-    eng.close()
-    out = st.summary()
-    out.update({"arrival_rate_per_s": rate, "rays_per_request": 1 << 20, "triangles": sc.n_tris,
-                "offered_mrays_per_s": round(n_requests * (1 << 20) / float(arr[-1]) / 1e6, 1)})
-    return out
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,6 +450,11 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a 256 MiB device-to-device copy, repeated: calibrates HBM byte counters")
+    ap.add_argument("--flop-probe", action="store_true",
+                    help="only a kernel of known FP32 operation count, repeated: calibrates the VALU FLOP counters")
+    ap.add_argument("--latency-frames", type=int, default=200, help="frames in the latency sample (p50/p99)")
+    ap.add_argument("--profile-bf", action="store_true",
+                    help="only the 320-triangle brute-force frame, --steps times (for a kernel profile)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only the 64-frame batch through the engine, --steps times (for a kernel profile)")
     args = ap.parse_args()
@@ -381,8 +479,35 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": max(args.steps, 1)}))
         return 0
+    if args.flop_probe:
+        P = C.CDLL(os.path.join(ROOT, "splat_amd", "libsplat_probe.so"))
+        P.splat_probe_fma.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]
+        P.splat_probe_fma.restype = C.c_int32
+        P.splat_probe_fma_flops.argtypes = [C.c_int64, C.c_int32]
+        P.splat_probe_fma_flops.restype = C.c_int64
+        n, iters = 1 << 24, 1024
+        out = torch.empty(n, dtype=torch.float32, device="cuda")
+        for _ in range(max(args.steps, 1)):
+            if P.splat_probe_fma(out.data_ptr(), n, iters, None) != 0:
+                raise RuntimeError("probe launch failed")
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "k_probe_fma", "threads": n, "iters": iters,
+                          "flops_per_launch": int(P.splat_probe_fma_flops(n, iters)), "launches": max(args.steps, 1)}))
+        return 0
     from splat_amd import api
 
+    if args.profile_bf:
+        from splat_amd import scene
+
+        sc = scene.scene_c1()
+        tris = api.to_device(sc.ftris)
+        fr = api.Frame(scene.camera_c1(seed=1), api.HIPBruteForceHitter(tris), tris,
+                       abi.run_params(2, sc.first_diffuse, math.nan, seed=2024), abi.lightmap_params(512, 512))
+        for _ in range(max(args.steps, 1)):
+            fr.trace()
+        torch.cuda.synchronize()
+        print(json.dumps({"frames": max(args.steps, 1), "workload": "320-triangle brute-force frame"}))
+        return 0
     if args.profile_batched:
         from splat_amd import serve
 
@@ -413,11 +538,20 @@ def main() -> int:
         evs[i + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    frame_ms = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(K)])
     value = R * K / wall / 1e6
     if args.profile:
         print(json.dumps({"frames": K, "ms_per_step": wall / K * 1e3}))
         return 0
+    # frame latency over a fixed sample of frames (not --steps): each frame's device time between
+    # events recorded before and after it on its stream, frames issued back to back
+    NL = max(args.latency_frames, 1)
+    lev = [torch.cuda.Event(enable_timing=True) for _ in range(NL + 1)]
+    lev[0].record()
+    for i in range(NL):
+        frame.trace()
+        lev[i + 1].record()
+    torch.cuda.synchronize()
+    frame_ms = np.array([lev[i].elapsed_time(lev[i + 1]) for i in range(NL)])
 
     # ---- instrumented frames: hitter and fused bounce-pass launches timed with events recorded
     # on the frame's own stream around each launch ----
@@ -453,15 +587,19 @@ def main() -> int:
     pass_s = float(np.mean([sum(x) for x in pass_ms])) / 1e3
 
     # ---- algorithmic work of the hitter (one counting frame, untimed) ----
-    stats = torch.zeros(4, dtype=torch.int64, device="cuda")
+    stats = torch.zeros(5, dtype=torch.int64, device="cuda")
     hitter.hs.d_stats = stats.data_ptr()
     frame.trace()
     torch.cuda.synchronize()
     hitter.hs.d_stats = None
-    box_tests, entered, alg_tests, exec_tests = (int(x) for x in stats.cpu().tolist())
+    box_tests, entered, alg_tests, exec_tests, box_exec = (int(x) for x in stats.cpu().tolist())
+    # algorithmic: each ray's own path (boxes of the nodes it entered, members of the leaves it
+    # entered); executed: every test the wavefronts ran, 64 lanes each
     flop_per_launch = (FLOP_PER_BOX_TEST * box_tests + FLOP_PER_TRI_TEST * alg_tests) / n_launch
+    flop_exec = (FLOP_PER_BOX_TEST * box_exec + FLOP_PER_TRI_TEST * exec_tests) / n_launch
     achieved = flop_per_launch / avg_launch_s / 1e12
     traffic, traffic_src = hitter_traffic()
+    pmc_flop, pmc_src = hitter_pmc_flops()
 
     del frame
     torch.cuda.empty_cache()
@@ -473,6 +611,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     render = None if args.no_served else render_frame()
     bvb = None if args.no_bv_build else bv_build()
+    large = None if args.no_served else large_frame()
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, params, lm)
     small = None if args.no_cpu_baseline else small_frame()
     out = {
@@ -500,9 +639,10 @@ def main() -> int:
         "ray_segments_per_s_M": round(sum(live[:D]) * K / wall / 1e6, 3),
         "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
+        "latency_frames": int(frame_ms.size),
         "roofline": {
             "kernel": "k_hit_bv (bounding-volume hitter)",
-            "bound": "mfma",  # the compute-bound category; the peak is the FP32 dense figure
+            "bound": "valu",  # per-ray FP32 vector ALU work (no matrix instructions); peak: FP32 vector
             "pipe": "fp32 VALU (per-ray scalar FP32 box and triangle tests; no matrix instructions)",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
@@ -513,6 +653,10 @@ def main() -> int:
             "traffic": traffic,
             "traffic_source": traffic_src,
             "flop_per_launch": flop_per_launch,
+            "flop_per_launch_executed": flop_exec,
+            "executed_tflops": round(flop_exec / avg_launch_s / 1e12, 3),
+            "pmc_flop_per_launch": pmc_flop,
+            "pmc_source": pmc_src,
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         },
         "rooflines": {
@@ -524,16 +668,19 @@ def main() -> int:
         },
         "hitter": {
             "ms_per_frame": round(hitter_frame_ms, 4),
-            "box_tests_per_frame": box_tests,
+            "box_tests_alg_per_frame": box_tests,
+            "box_tests_exec_per_frame": box_exec,
             "tri_tests_alg_per_frame": alg_tests,
             "tri_tests_exec_per_frame": exec_tests,
-            "simd_efficiency": round(alg_tests / max(exec_tests, 1), 4),
+            "simd_efficiency_tri": round(alg_tests / max(exec_tests, 1), 4),
+            "simd_efficiency_box": round(box_tests / max(box_exec, 1), 4),
         },
         "batched": batched,
         "served": serve_out,
         "served_1mp": serve_1mp,
         "render": render,
         "bv_build": bvb,
+        "large_scene": large,
         "cpu_baseline": cpu,
         "small_frame": small,
     }

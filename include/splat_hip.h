@@ -133,10 +133,13 @@ typedef struct splat_hitset {
   int32_t tree_levels;   /* 1 + levels of the tree over the BVs; 0 = no tree (BV only)    */
   int32_t top_bvs;       /* leading BVs whose members every ray tests directly (BV only) */
   int32_t top_recs;      /* their records: d_recs[0, top_recs)                            */
-  /* Optional device uint64[4] counters, accumulated by every hitter launch when non-NULL:
-   * [0] ray-box slab tests, [1] (ray, box) pairs entered, [2] ray-triangle tests of the
-   * algorithm (brute force: every live ray x every triangle; BV: the members of the boxes each
-   * ray entered), [3] ray-triangle tests the wavefronts executed. NULL costs nothing. */
+  /* Optional device uint64[5] counters, accumulated by every hitter launch when non-NULL:
+   * [0] ray-box slab tests on each ray's own path (the root of its split, then the children of
+   * every node that ray entered; every box of a flat scan), [1] (ray, leaf) pairs entered,
+   * [2] ray-triangle tests of the algorithm (brute force: every live ray x every triangle; BV:
+   * the members of the leaves each ray entered, and the large-triangle list), [3] ray-triangle
+   * and [4] ray-box tests the wavefronts executed (64 lanes per wave test, live or not). NULL
+   * costs nothing. */
   uint64_t* d_stats;
 } splat_hitset_t;
 

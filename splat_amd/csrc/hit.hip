@@ -178,7 +178,10 @@ __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict_
     atomicAdd(stats + 0, st.n_box);
     atomicAdd(stats + 1, st.n_enter);
     atomicAdd(stats + 2, st.n_alg);
-    if ((threadIdx.x & 63) == 0) atomicAdd(stats + 3, 64ull * st.n_exec);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(stats + 3, 64ull * st.n_exec);
+      atomicAdd(stats + 4, 64ull * st.n_box_exec);
+    }
   }
 }
 

@@ -108,8 +108,13 @@ __device__ __forceinline__ void scan_records16(const TriRec* __restrict__ recs, 
   __builtin_amdgcn_wave_barrier();  // the next block's writes stay after these reads
 }
 
+// Counters of a STATS build, per lane: n_box -- ray-box tests on the ray's own path (the boxes
+// a per-ray traversal of the same structure tests: the root, then the children of every node the
+// ray entered; every box of a flat scan); n_enter -- leaves the ray entered; n_alg -- triangle
+// tests of those leaves (and of the large-triangle list). Per wave (the same in every lane):
+// n_exec -- triangle tests and n_box_exec -- box tests the wave executed (each for all 64 lanes).
 struct HitStats {
-  unsigned long long n_box = 0, n_enter = 0, n_alg = 0, n_exec = 0;
+  unsigned long long n_box = 0, n_enter = 0, n_alg = 0, n_exec = 0, n_box_exec = 0;
 };
 
 // (t, index) key of a miss: (Inf, 1)
@@ -186,6 +191,7 @@ __device__ __forceinline__ void tree_walk_lanes(const RayLanes& L, bool valid, f
     // walk until this lane holds a leaf it enters, or is done
     int32_t leaf = -1;
     while (__builtin_amdgcn_ballot_w64(!done && leaf < 0) != 0) {
+      if constexpr (STATS) st.n_box_exec += 1ull;
       if (!done && leaf < 0) {
         const float4 lo = nodes[2 * n], hi = nodes[2 * n + 1];
         const int32_t code = __float_as_int(lo.w);
@@ -245,6 +251,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
   unsigned long long& n_enter = st.n_enter;
   unsigned long long& n_alg = st.n_alg;
   unsigned long long& n_exec = st.n_exec;
+  unsigned long long& n_box_exec = st.n_box_exec;
   auto safe_inv = [](float d) {
     float a = fabsf(d) < 1e-30f ? copysignf(1e-30f, d) : d;
     return 1.0f / a;
@@ -286,10 +293,13 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     }
     for (int32_t m = 0; m < nt; m += 16) scan_records16(S.recs + m, nt - m < 16 ? nt - m : 16, L, best, bi);
   }
-  auto scan_leaf = [&](int32_t b) {
+  auto scan_leaf = [&](int32_t b, bool in_group) {
     const float4 blo = boxes[2 * b], bhi = boxes[2 * b + 1];
     const bool enter = valid && slab_enter(blo, bhi, L, ix, iy, iz, best);
-    if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+    if constexpr (STATS) {
+      n_box += in_group ? 1ull : 0ull;
+      n_box_exec += 1ull;
+    }
     if (__builtin_amdgcn_ballot_w64(enter) == 0) return;
     scan_members(b, enter);
   };
@@ -301,10 +311,13 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       merge(fetch());
       const float4 glo = groups[2 * g], ghi = groups[2 * g + 1];
       const bool enter = valid && slab_enter(glo, ghi, L, ix, iy, iz, best);
-      if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+      if constexpr (STATS) {
+        n_box += valid ? 1ull : 0ull;
+        n_box_exec += 1ull;
+      }
       if (__builtin_amdgcn_ballot_w64(enter) == 0) continue;
       const int32_t b1 = (g + 1) * gsize < n_bv ? (g + 1) * gsize : n_bv;
-      for (int32_t b = g * gsize; b < b1; ++b) scan_leaf(b);
+      for (int32_t b = g * gsize; b < b1; ++b) scan_leaf(b, enter);
     }
   } else if constexpr (MODE == BV_TREE) {
     const cptr<float4> nodes = as_const(S.nodes);
@@ -338,9 +351,15 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       // the hits found meanwhile.
       const int32_t root = n_sub + split;
       int32_t stack = 0, sp = 0;  // lane k of `stack`: entry k
+      // STATS: lane k of mlo/mhi holds the mask of the lanes that entered entry k
+      uint32_t mlo = 0, mhi = 0;
       const int32_t my_lane = threadIdx.x & 63;
-      auto push = [&](int32_t v) {
+      auto push = [&](int32_t v, uint64_t m) {
         stack = my_lane == sp ? v : stack;
+        if constexpr (STATS) {
+          mlo = my_lane == sp ? (uint32_t)m : mlo;
+          mhi = my_lane == sp ? (uint32_t)(m >> 32) : mhi;
+        }
         ++sp;
       };
       // entry distance of this lane's ray into a node box: tmin where the slab intervals overlap,
@@ -348,7 +367,6 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       // the slab test, and stays exact later for a nearer best (tmin and tmax do not depend on it)
       auto lane_t = [&](const float4& lo, const float4& hi) -> float {
         if (!valid || (__float_as_int(lo.w) & TREE_EMPTY)) return __builtin_nanf("");
-        if constexpr (STATS) n_box += 1ull;
         float tmin;
         return slab_entry(lo, hi, L, ix, iy, iz, INFINITY, tmin) ? tmin : __builtin_nanf("");
       };
@@ -367,16 +385,32 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       };
       {
         const float t = lane_t(nodes[2 * root], nodes[2 * root + 1]);
-        if (__builtin_amdgcn_ballot_w64(t <= best) != 0) {
+        if constexpr (STATS) {
+          n_box += valid ? 1ull : 0ull;
+          n_box_exec += 1ull;
+        }
+        const uint64_t m = __builtin_amdgcn_ballot_w64(t <= best);
+        if (m != 0) {
           if (root >= leaf0) visit(root, t);
-          else push(root);
+          else push(root, m);
         }
       }
       while (sp > 0) {
         const int32_t n = __builtin_amdgcn_readlane(stack, --sp);
+        // STATS: did this lane's ray enter n (its own path continues below n)?
+        bool in_n = false;
+        if constexpr (STATS) {
+          const uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int32_t)mhi, sp) << 32) |
+                             (uint32_t)__builtin_amdgcn_readlane((int32_t)mlo, sp);
+          in_n = ((m >> my_lane) & 1ull) != 0;
+        }
         if (2 * n >= leaf0) {  // the children are leaves: nearer first
           const float ta = lane_t(nodes[4 * n], nodes[4 * n + 1]);
           const float tb = lane_t(nodes[4 * n + 2], nodes[4 * n + 3]);
+          if constexpr (STATS) {
+            n_box += in_n ? 2ull : 0ull;
+            n_box_exec += 2ull;
+          }
           const bool wa = __builtin_amdgcn_ballot_w64(ta <= best) != 0, wb = __builtin_amdgcn_ballot_w64(tb <= best) != 0;
           const bool b_first = wb && (!wa || near_half(__float_as_int(nodes[2 * n].w)) == 1);
           visit(2 * n + (b_first ? 1 : 0), b_first ? tb : ta);
@@ -387,8 +421,16 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
         const int32_t g = 8 * n;
         const float t0 = lane_t(nodes[g], nodes[g + 1]), t1 = lane_t(nodes[g + 2], nodes[g + 3]),
                     t2 = lane_t(nodes[g + 4], nodes[g + 5]), t3 = lane_t(nodes[g + 6], nodes[g + 7]);
-        const bool w0 = __builtin_amdgcn_ballot_w64(t0 <= best) != 0, w1 = __builtin_amdgcn_ballot_w64(t1 <= best) != 0,
-                   w2 = __builtin_amdgcn_ballot_w64(t2 <= best) != 0, w3 = __builtin_amdgcn_ballot_w64(t3 <= best) != 0;
+        const uint64_t m0 = __builtin_amdgcn_ballot_w64(t0 <= best), m1 = __builtin_amdgcn_ballot_w64(t1 <= best),
+                       m2 = __builtin_amdgcn_ballot_w64(t2 <= best), m3 = __builtin_amdgcn_ballot_w64(t3 <= best);
+        const bool w0 = m0 != 0, w1 = m1 != 0, w2 = m2 != 0, w3 = m3 != 0;
+        if constexpr (STATS) {
+          // a per-ray walk tests n's two children, then the two children of each it enters
+          const bool e0 = in_n && lane_t(nodes[4 * n], nodes[4 * n + 1]) <= best;
+          const bool e1 = in_n && lane_t(nodes[4 * n + 2], nodes[4 * n + 3]) <= best;
+          n_box += (in_n ? 2ull : 0ull) + (e0 ? 2ull : 0ull) + (e1 ? 2ull : 0ull);
+          n_box_exec += 4ull;
+        }
         const uint32_t wmask = (w0 ? 1u : 0u) | (w1 ? 2u : 0u) | (w2 ? 4u : 0u) | (w3 ? 8u : 0u);
         if (wmask == 0) continue;
         // visiting rank of grandchild j = 2c + h (child c, half h): nearer child first, then its
@@ -420,10 +462,11 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
           if ((wmask >> i2) & 1u) visit(4 * n + i2, pick(i2, t0, t1, t2, t3));
           if ((wmask >> i3) & 1u) visit(4 * n + i3, pick(i3, t0, t1, t2, t3));
         } else {
-          if ((wmask >> i3) & 1u) push(4 * n + i3);
-          if ((wmask >> i2) & 1u) push(4 * n + i2);
-          if ((wmask >> i1) & 1u) push(4 * n + i1);
-          if ((wmask >> i0) & 1u) push(4 * n + i0);
+          auto mask_of = [&](int32_t j) { return j == 0 ? m0 : j == 1 ? m1 : j == 2 ? m2 : m3; };
+          if ((wmask >> i3) & 1u) push(4 * n + i3, mask_of(i3));
+          if ((wmask >> i2) & 1u) push(4 * n + i2, mask_of(i2));
+          if ((wmask >> i1) & 1u) push(4 * n + i1, mask_of(i1));
+          if ((wmask >> i0) & 1u) push(4 * n + i0, mask_of(i0));
         }
       }
     }
@@ -440,7 +483,10 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
         if (b < n_bv) {
           const float4 lo = boxes[2 * b], hi = boxes[2 * b + 1];
           const bool e = valid && slab_enter(lo, hi, L, ix, iy, iz, best);
-          if constexpr (STATS) n_box += valid ? 1ull : 0ull;
+          if constexpr (STATS) {
+            n_box += valid ? 1ull : 0ull;
+            n_box_exec += 1ull;
+          }
           if (__builtin_amdgcn_ballot_w64(e) != 0) wave_mask |= 1u << j;
           lane_mask |= (e ? 1u : 0u) << j;
         }
