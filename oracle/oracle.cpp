@@ -580,7 +580,8 @@ template <class T> RayT<T> final_evolution_one(const RayT<T>& r, int64_t i, cons
   return out;
 }
 
-// piecewise-linear response (src/rgb_spectrum.jl:116-145): on segment m, slope*(l - x_{m+1}) + y_{m+1}
+// piecewise-linear response (src/rgb_spectrum.jl:3-32, normalised as :116-130): on segment m
+// (x_m <= l < x_{m+1}, the end segments extended), slope*(l - x_{m+1}) + y_{m+1}
 template <class T> inline T curve_eval(const splat_lightmap_params_t& lm, int c, T lam) {
   int n = lm.curve_n[c];
   int m = 0;

@@ -160,9 +160,10 @@ def upres(n: int) -> np.ndarray:
     return (k / np.float32(n) - np.float32(0.5)) - (np.float32(1.0) / (np.float32(2.0) * np.float32(n)))
 
 
-# Piecewise-linear retina response curves of src/rgb_spectrum.jl:147-200 (the same table; it is
-# the reference's data, kept as is). Normalised by the sum over integer lambda 400..700
-# (src/rgb_spectrum.jl:229-243).
+# Piecewise-linear retina response curves of src/rgb_spectrum.jl:34-87 (the same table; it is
+# the reference's data, kept as is), evaluated as its `piecewise` does (src/rgb_spectrum.jl:3-32:
+# segment m for x_m <= lambda < x_{m+1}, the end segments extended). Normalised by the sum over
+# integer lambda 400..700 (src/rgb_spectrum.jl:116-130).
 _RETINA = {
     "red": [(399, 0), (400, 0.0647036), (420, 0.147492), (438, 0.195683), (460, 0.162951), (466, 0),
             (534, 0), (538, 0.243127), (571, 0.798209), (593, 0.980253), (602, 0.999977), (610, 0.98481),
