@@ -61,18 +61,23 @@ class Hip:
             self.h.hipEventDestroy(e)
 
 
-def build_workload(device="cuda"):
-    from splat_amd import abi, api, bv, scene
+def build_workload(device="cuda", tune=True, tune_cache=None):
+    """The 1 MP frame; its BV tree is built on the device with the leaf depth and walk mode the
+    autotuner picks on this frame (whole frames timed per candidate)."""
+    from splat_amd import abi, api, autotune, scene
 
     sc = scene.scene_c2(seed=2)
     cam = scene.camera_c2(seed=2, jitter=0.025)
-    bvs, mem, n_top = bv.tree_partition(sc.ftris)
     tris_d = api.to_device(sc.ftris, device)
-    hitter = api.HIPBVHitter(tris_d, bvs, mem, group_size=0, n_top=n_top)
     params = abi.run_params(4, sc.first_diffuse, math.nan, seed=2024)
     lm = abi.lightmap_params(1024, 1024)
+    if tune:
+        ch = autotune.tune_device_bv(tris_d, cam, params, lm, cache_path=tune_cache)
+        hitter, tuned = ch["hitter"], {k: v for k, v in ch.items() if k != "hitter"}
+    else:
+        hitter, tuned = api.HIPBVHitter.build_on_device(tris_d), None
     frame = api.Frame(cam, hitter, tris_d, params, lm, device)
-    return sc, cam, (bvs, mem), hitter, params, lm, frame
+    return sc, cam, tuned, hitter, params, lm, frame
 
 
 def hitter_traffic():
@@ -229,14 +234,16 @@ def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
 def batched_vs_plain(reps: int = 3):
     """64 mixed frames (256^2-1024^2 rays, icosphere subdiv 2-5 + plane, 4 bounces): all in one
     engine batch (every frame's work items in shared launches, one graph replay per step) versus the
-    same frames traced one frame per launch sequence."""
+    same frames traced one frame per launch sequence. Both keep each frame's light map only (the
+    served product): no final rays or hit indices on either side."""
     import torch
 
     from splat_amd import api, serve
 
     scenes, reqs = serve.batched_workload(64)
     rays = sum(r.rays for r in reqs)
-    frames = [api.Frame(r.cam, scenes[r.scene].hitter, scenes[r.scene].tris, r.params, r.lm) for r in reqs]
+    frames = [api.Frame(r.cam, scenes[r.scene].hitter, scenes[r.scene].tris, r.params, r.lm, keep_rays=False)
+              for r in reqs]
     for f in frames:  # warm-up
         f.trace()
     torch.cuda.synchronize()
@@ -260,6 +267,7 @@ def batched_vs_plain(reps: int = 3):
     eng.close()
     tp, tb = min(plain), min(batched)
     return {"frames": len(reqs), "rays": rays, "plain_mrays_per_s": round(rays / tp / 1e6, 3),
+            "autotuned": [{k: s.tuned[k] for k in ("depth", "walk")} if s.tuned else None for s in scenes],
             "batched_mrays_per_s": round(rays / tb / 1e6, 3), "batched_vs_plain": round(tp / tb, 3),
             "plain_s": round(tp, 4), "batched_s": round(tb, 4), "steps_per_batch": st.steps}
 
@@ -295,7 +303,8 @@ def served(n_requests: int = 120, rates=(40.0, 150.0, 300.0, 600.0, 1200.0)):
         sweep.append(_sweep_point(st, reqs, rate, "small"))
     eng.close()
     out = dict(sweep[0])
-    out.update({"hbm_budget_gib": 96, "triangles": [s.sc.n_tris for s in scenes], "sweep": sweep})
+    out.update({"hbm_budget_gib": 96, "triangles": [s.sc.n_tris for s in scenes], "sweep": sweep,
+                "autotuned": [{k: s.tuned[k] for k in ("depth", "walk")} if s.tuned else None for s in scenes]})
     return out
 
 
@@ -453,6 +462,8 @@ def main() -> int:
     ap.add_argument("--flop-probe", action="store_true",
                     help="only a kernel of known FP32 operation count, repeated: calibrates the VALU FLOP counters")
     ap.add_argument("--latency-frames", type=int, default=200, help="frames in the latency sample (p50/p99)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner JSON cache (a profiling run given the bench's cache traces no tuning frames)")
     ap.add_argument("--profile-bf", action="store_true",
                     help="only the 320-triangle brute-force frame, --steps times (for a kernel profile)")
     ap.add_argument("--profile-batched", action="store_true",
@@ -522,7 +533,7 @@ def main() -> int:
         eng.close()
         print(json.dumps({"batches": len(walls), "batched_s": walls}))
         return 0
-    sc, cam, bvmem, hitter, params, lm, frame = build_workload()
+    sc, cam, tuned, hitter, params, lm, frame = build_workload(tune_cache=args.tune_cache)
     R = frame.R
     for _ in range(args.warmup):
         frame.trace()
@@ -629,8 +640,10 @@ def main() -> int:
         "data": "synthetic: seeded procedural scene and pinhole rays (no dataset)",
         "config": {
             "workload": "1 MP light-map frame: 2 displaced icospheres (subdiv 4, 10,240 glass tris) + diffuse plane, "
-                        "BV hitter (the 2 ground triangles tested directly, binary tree over 2^10 object-median leaves of the rest), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
-                        "light map; plain path (one frame per launch sequence)",
+                        "BV hitter (the 2 ground triangles tested directly, binary tree over 2^%d object-median leaves of "
+                        "the rest, built on the device), 4 bounces, dual-number differentials, 1024x1024x3 f32 "
+                        "light map; plain path (one frame per launch sequence)" % hitter.depth,
+            "autotuned": tuned,
             "rays_per_frame": R,
             "triangles": sc.n_tris,
             "bounces": params.depth,

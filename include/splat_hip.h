@@ -367,7 +367,10 @@ int32_t splat_run_evolution(splat_adray_t* rays, int64_t R, const splat_hitset_t
                             int64_t* d_live_out, void* stream);
 
 /* One full light-map frame: pinhole rays -> run_evolution -> light map, all on the device.
- * `tex` is cleared first. `rays`/`hit_idx` must hold cam's R rays; ws from splat_run_ws_bytes. */
+ * `tex` is cleared first. `rays`/`hit_idx` receive cam's R final rays and hit indices, as
+ * splat_run_evolution leaves them; both NULL: only the light map is kept (the reference's forward
+ * pass discards its rays after the splat), and the writes of retired rays and the realigning hit
+ * that only feed those outputs are skipped. ws from splat_run_ws_bytes. */
 int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64_t R,
                           const splat_hitset_t* hs, const splat_ftri_t* tris,
                           int32_t n_tris_len, int32_t* hit_idx, const splat_run_params_t* params,

@@ -106,18 +106,21 @@ class HIPBVHitter:
                                             self.ws.numel(), L.ref(self.hs), L.stream_handle()), "prepare_bv")
 
     @classmethod
-    def build_on_device(cls, tris, depth: int | None = None, big_factor: float = 16.0, max_big: int = 32):
+    def build_on_device(cls, tris, depth: int | None = None, big_factor: float = 16.0, max_big: int = 32,
+                        walk: int = 0):
         """The tree hitter's BVs built on the device from the triangles alone
         (splat_hitset_build_bv): the memberships bv.tree_partition(depth, big_factor, max_big)
         makes on the host, with the large-triangle list always present (possibly empty) and
-        empty leaves kept. depth defaults to bv.tree_depth of all the triangles."""
+        empty leaves kept. depth defaults to bv.tree_depth of all the triangles. `walk` is the
+        tree walk mode (HitSet.tune_waves: 0 waves together, 1 per wave by direction octant, 2 per
+        lane; + 4 never the flat tail scan), kept across rebuilds."""
         L.require_gpu()
         _check_rows(tris, abi.FTRI.itemsize, "tris")
         self = cls.__new__(cls)
         self.bvs = self.counts = self.mat = self.stride = None
         N = _nrows(tris)
         self.depth = int(depth) if depth is not None else bvmod.tree_depth(N - 1)
-        self.big_factor, self.max_big = float(big_factor), int(max_big)
+        self.big_factor, self.max_big, self.walk = float(big_factor), int(max_big), int(walk)
         nbytes = L.lib().splat_hitset_build_bv_bytes(N, self.depth)
         if nbytes == 0:
             raise ValueError(f"device BV build: unsupported size (N={N}, depth={self.depth})")
@@ -134,6 +137,7 @@ class HIPBVHitter:
         L.check(L.lib().splat_hitset_build_bv(L.ptr(tris), _nrows(tris), self.depth, self.big_factor, self.max_big,
                                               L.ptr(self.ws), self.ws.numel(), L.ref(self.hs), L.stream_handle()),
                 "build_bv")
+        self.hs.tune_waves = self.walk
 
 
 class HIPTracer:
@@ -391,16 +395,18 @@ def trace_(**kw):
 
 class Frame:
     """Device-resident state of one light-map frame: rays, tracer arrays, workspace, light map.
-    `trace(...)` runs pinhole -> run_evolution! -> continuum_light_map2! with no host sync."""
+    `trace(...)` runs pinhole -> run_evolution! -> continuum_light_map2! with no host sync.
+    keep_rays=False keeps only the light map (rays and hit_idx are None): the final rays are not
+    written and the realigning hit is skipped, as nothing reads them."""
 
     def __init__(self, cam: abi.Pinhole, hitter, tris, params: abi.RunParams, lm: abi.LightMapParams,
-                 device: str = "cuda"):
+                 device: str = "cuda", keep_rays: bool = True):
         torch = _torch()
         L.require_gpu()
         self.cam, self.hitter, self.tris, self.params, self.lm = cam, hitter, tris, params, lm
         self.R = cam.height * cam.width * cam.samples_per_pixel
-        self.rays = torch.empty((self.R, abi.ADRAY.itemsize), dtype=torch.uint8, device=device)
-        self.hit_idx = torch.empty(self.R, dtype=torch.int32, device=device)
+        self.rays = torch.empty((self.R, abi.ADRAY.itemsize), dtype=torch.uint8, device=device) if keep_rays else None
+        self.hit_idx = torch.empty(self.R, dtype=torch.int32, device=device) if keep_rays else None
         self.tex = torch.zeros((lm.channels, lm.height, lm.width), dtype=torch.float32, device=device)
         self.ws = _ws(L.lib().splat_run_ws_bytes(self.R))
 

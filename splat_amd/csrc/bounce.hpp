@@ -146,7 +146,8 @@ __device__ __forceinline__ void splat_tile(BounceLDS& lds, bool has, int32_t sx,
 }
 
 // The bounce pass of one tile (see the top of this file). off_a / off_r: where the tile's ACTIVE
-// and other rays start in out_active / out_rest; hit_rest gets the hit index of the others;
+// and other rays start in out_active / out_rest; hit_rest gets the hit index of the others (either
+// may be null: a frame whose retired rays nobody reads -- only its light map -- skips them);
 // keys_next (nullable) gets the miss key (Inf, 1) at each ACTIVE ray's new position, ready for
 // the next bounce's hitter. CAM: the tile is generated from the pinhole (rays base, base + 1,
 // ...) instead of read from `in`.
@@ -231,11 +232,11 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
   }
   if (rest) {
     order[na + pr + rr] = (int16_t)tid;
-    hit_rest[off_r + pr + rr] = i;
+    if (hit_rest) hit_rest[off_r + pr + rr] = i;
   }
   __syncthreads();
   write_words(stage, order, 0, na * WORDS, reinterpret_cast<uint2*>(out_active + off_a));
-  write_words(stage, order, na * WORDS, nvalid * WORDS, reinterpret_cast<uint2*>(out_rest + off_r));
+  if (out_rest) write_words(stage, order, na * WORDS, nvalid * WORDS, reinterpret_cast<uint2*>(out_rest + off_r));
   if (one_sample) splat_tile(lds, has, sx, sy, sv, P, lm);
   __syncthreads();  // the LDS stage is reused by the caller's next tile
 }

@@ -132,7 +132,9 @@ int32_t check_optics(const splat_optics_t* o) {
 // with cam != nullptr (and depth >= 1), are that pinhole's rays, generated where the first
 // bounce needs them. With lm != nullptr, rays that become DIFFUSE are final-evolved and splatted
 // into tex in the loop (the light map of the frame), otherwise they are left as run_evolution!
-// leaves them.
+// leaves them. A light-map frame with neither final_rays nor hit_final keeps no rays: retired
+// rays are not written, and the realigning hit and final copy (which only feed those outputs)
+// are skipped.
 int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris,
                  int32_t* hit_final, const splat_run_params_t* p, const RunWs& w, hipStream_t s,
                  const splat_lightmap_params_t* lm = nullptr, float* tex = nullptr,
@@ -180,6 +182,7 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
     std::swap(kc, kn);
   }
   const int64_t* live = &w.counters[p->depth];
+  if (!final_rays && !hit_final) return launch_status();  // a light map only: nothing to realign
   // the reference's realigning next_hit! (src/tracer.jl:280)
   if (!bv) {
     mark(2 * p->depth);
@@ -194,9 +197,12 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
   return launch_status();
 }
 
+// (outputs_optional: a light-map frame may keep neither its rays nor their hit indices, both NULL)
 int32_t check_run(const splat_adray_t* rays, int64_t R, const splat_hitset_t* hs, const splat_ftri_t* tris, int32_t N,
-                  const int32_t* hit_idx, const splat_run_params_t* p, void* ws, size_t ws_bytes) {
-  if (!rays || R < 0 || !tris || N < 1 || !hit_idx || !p || !ws) return SPLAT_ERR_INVALID_ARG;
+                  const int32_t* hit_idx, const splat_run_params_t* p, void* ws, size_t ws_bytes,
+                  bool outputs_optional = false) {
+  const bool outputs_ok = (rays && hit_idx) || (outputs_optional && !rays && !hit_idx);
+  if (!outputs_ok || R < 0 || !tris || N < 1 || !p || !ws) return SPLAT_ERR_INVALID_ARG;
   if (int32_t e = check_hitset(hs)) return e;
   if (int32_t e = check_optics(&p->optics)) return e;
   if (p->depth < 0 || p->depth > MAX_DEPTH) return SPLAT_ERR_UNSUPPORTED;
@@ -637,7 +643,7 @@ int32_t splat_trace_frame(const splat_pinhole_t* cam, splat_adray_t* rays, int64
                           const splat_ftri_t* tris, int32_t N, int32_t* hit_idx, const splat_run_params_t* p,
                           const splat_lightmap_params_t* lm, float* tex, void* ws, size_t ws_bytes, void* stream) {
   if (!cam || !tex) return SPLAT_ERR_INVALID_ARG;
-  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes)) return e;
+  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes, true)) return e;
   if (int32_t e = check_lightmap(lm)) return e;
   if ((int64_t)cam->height * cam->width * cam->samples_per_pixel != R || !(cam->fov_half_sin > 0.f))
     return SPLAT_ERR_INVALID_ARG;
@@ -656,7 +662,7 @@ int32_t splat_trace_lights(const splat_rect_light_t* lights, int32_t n, const sp
                            int32_t N, int32_t* hit_idx, const splat_run_params_t* p,
                            const splat_lightmap_params_t* lm, float* tex, void* ws, size_t ws_bytes, void* stream) {
   if (!lp || !tex) return SPLAT_ERR_INVALID_ARG;
-  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes)) return e;
+  if (int32_t e = check_run(rays, R, hs, tris, N, hit_idx, p, ws, ws_bytes, true)) return e;
   if (int32_t e = check_lightmap(lm)) return e;
   const int64_t total = splat_lights_ray_count(lights, n, lp->upscale);
   if (total < 0) return (int32_t)total;

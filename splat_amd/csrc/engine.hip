@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <array>
 #include <map>
 #include <vector>
 
@@ -36,9 +37,9 @@ constexpr int HIT_ITEM = 64;     // rays per hitter work item (one wave)
 constexpr int CLEAR_ITEM = 4096;  // light-map floats per clear item
 
 struct SlotDev {
-  splat_adray_t* F;        // final rays (the frame's output order)
+  splat_adray_t* F;        // final rays (the frame's output order); null: not kept
   splat_adray_t* S[2];     // live rays, ping-pong by stage parity
-  int32_t* hit_final;
+  int32_t* hit_final;      // their hit indices; null: not kept
   uint64_t* keys[2];       // packed (t, index) nearest hit of each live ray, by stage parity
   int32_t* tile_counts;
   int32_t* tile_offsets;
@@ -48,6 +49,7 @@ struct SlotDev {
   int64_t R;
   int64_t tex_n;
   int32_t n_tris, first_diffuse, depth;
+  int32_t last;            // the frame's last stage: depth, or depth - 1 when it keeps only its light map
   float force_rand;
   uint64_t seed;
   splat_optics_t optics;
@@ -378,10 +380,13 @@ __global__ __launch_bounds__(TILE) void k_b_final(EngineDev E) {
     const int64_t A = E.counters[(int64_t)s * (MAXD + 2) + st];
     const int64_t base = (int64_t)(i - P[s]) * TILE;
     const int64_t nw = (A - base < TILE ? A - base : TILE) * W;
-    const uint2* src = reinterpret_cast<const uint2*>(D.S[st & 1] + base);
-    uint2* dst = reinterpret_cast<uint2*>(D.F + base);
-    for (int64_t k = threadIdx.x; k < nw; k += TILE) dst[k] = src[k];
-    if (base + threadIdx.x < A) D.hit_final[base + threadIdx.x] = key_index(D.keys[st & 1][base + threadIdx.x]);
+    if (D.F) {
+      const uint2* src = reinterpret_cast<const uint2*>(D.S[st & 1] + base);
+      uint2* dst = reinterpret_cast<uint2*>(D.F + base);
+      for (int64_t k = threadIdx.x; k < nw; k += TILE) dst[k] = src[k];
+    }
+    if (D.hit_final && base + threadIdx.x < A)
+      D.hit_final[base + threadIdx.x] = key_index(D.keys[st & 1][base + threadIdx.x]);
   }
 }
 
@@ -389,7 +394,7 @@ __global__ void k_b_finish(EngineDev E) {
   for (int s = threadIdx.x; s < E.n_slots; s += blockDim.x) {
     const int32_t st = E.stage[s];
     if (st < 0) continue;
-    E.stage[s] = st >= E.slots[s].depth ? -1 : st + 1;
+    E.stage[s] = st >= E.slots[s].last ? -1 : st + 1;
   }
 }
 
@@ -440,6 +445,7 @@ struct HostSlot {
   bool used = false;
   int32_t stage = -1;  // host mirror: stage the slot runs at in the next step
   int32_t depth = 0;
+  int32_t last = 0;  // last stage (SlotDev::last)
   int64_t R = 0;
   int64_t tag = 0;
   std::vector<Block> blocks;
@@ -450,6 +456,9 @@ struct Pending {
   std::vector<int32_t> finishing;  // slots whose last stage ran in this step
 };
 
+// arena bytes of a frame of R rays (two live-ray buffers, two key buffers, tile counts and
+// offsets); with_outputs adds its output arrays (final rays and hit indices), which the caller
+// allocates, for a budget that covers both
 size_t frame_bytes(int64_t R, bool with_outputs) {
   int64_t tiles = (R + TILE - 1) / TILE;
   size_t b = 2 * al((size_t)R * sizeof(splat_adray_t)) + 2 * al((size_t)R * 8) + 2 * al((size_t)tiles * 4);
@@ -480,7 +489,7 @@ struct splat_engine {
   std::vector<HostSlot> slots;
   std::deque<Pending> pending;
   std::vector<hipEvent_t> free_events;
-  std::map<int64_t, hipGraphExec_t> graphs;  // one captured step per set of grid sizes (StepGrids::key)
+  std::map<std::array<int64_t, 4>, hipGraphExec_t> graphs;  // one captured step per set of grid sizes
   hipStream_t cap_stream = nullptr;  // graphs are captured here (the null stream cannot capture)
   int use_graph = 1;
 };
@@ -493,12 +502,21 @@ namespace {
 // per wave over a quarter of it; the first-stage and last-stage kernels stride over grids for the
 // frames at those stages (1 when none is).
 struct StepGrids {
-  int64_t hit_blocks, tile_blocks, first_blocks, final_blocks;  // powers of two
-  int64_t key() const {
-    auto lg = [](int64_t v) { return (int64_t)__builtin_ctzll((unsigned long long)v); };
-    return lg(hit_blocks) | (lg(tile_blocks) << 6) | (lg(first_blocks) << 12) | (lg(final_blocks) << 18);
-  }
+  int64_t hit_blocks, tile_blocks, first_blocks, final_blocks;  // grid classes (grid_class)
+  std::array<int64_t, 4> key() const { return {hit_blocks, tile_blocks, first_blocks, final_blocks}; }
 };
+
+// The smallest grid class >= n: 1, 2, 3, then m * 2^k with m in {4, 5, 6, 7}. Four classes per
+// power of two bound the dispatch of workgroups past the items to 25% (a power-of-two grid
+// dispatched up to twice the items), with a few graph variants per octave.
+int64_t grid_class(int64_t n) {
+  if (n <= 3) return n < 1 ? 1 : n;
+  int64_t k = 0;
+  while ((7LL << k) < n) ++k;  // now 7 * 2^k >= n
+  for (int64_t m = 4; m <= 7; ++m)
+    if ((m << k) >= n) return m << k;
+  return 7LL << k;
+}
 
 int32_t launch_step_kernels(splat_engine* e, const StepGrids& g, hipStream_t s) {
   const EngineDev& E = e->dev;
@@ -613,9 +631,8 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   if (slot < 0) return SPLAT_ERR_BUDGET;
   HostSlot& hsl = e->slots[slot];
   const int64_t tiles = (R + TILE - 1) / TILE;
-  std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 8, tiles * 4, tiles * 4, R * 8};
-  if (!rq->rays_out) sizes.push_back(R * (int64_t)sizeof(splat_adray_t));
-  if (!rq->hit_out) sizes.push_back(R * 4);
+  const std::vector<int64_t> sizes = {R * (int64_t)sizeof(splat_adray_t), R * (int64_t)sizeof(splat_adray_t), R * 8,
+                                      tiles * 4, tiles * 4, R * 8};
   std::vector<Block> got;
   for (int64_t n : sizes) {
     int64_t off;
@@ -634,9 +651,12 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   D.tile_counts = reinterpret_cast<int32_t*>(at(3));
   D.tile_offsets = reinterpret_cast<int32_t*>(at(4));
   D.keys[1] = reinterpret_cast<uint64_t*>(at(5));
-  size_t k = 6;
-  D.F = rq->rays_out ? rq->rays_out : reinterpret_cast<splat_adray_t*>(at(k++));
-  D.hit_final = rq->hit_out ? rq->hit_out : reinterpret_cast<int32_t*>(at(k++));
+  D.F = rq->rays_out;
+  D.hit_final = rq->hit_out;
+  // a frame that keeps only its light map ends with its last bounce: the final stage (realigning
+  // hit, copy of the live rays) only feeds the outputs it does not keep
+  const bool map_only = !rq->rays_out && !rq->hit_out;
+  D.last = map_only && rq->depth > 0 ? rq->depth - 1 : rq->depth;
   D.tex = rq->tex;
   D.tris = rq->tris;
   D.scene = bv_scene(&rq->hs);
@@ -657,6 +677,7 @@ int32_t splat_engine_admit(splat_engine_t* e, const splat_frame_request_t* rq, v
   hsl.used = true;
   hsl.stage = 0;
   hsl.depth = rq->depth;
+  hsl.last = D.last;
   hsl.R = R;
   hsl.tag = rq->tag;
   hsl.blocks = got;
@@ -709,18 +730,17 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
     e->admits.clear();
     e->up_lo = e->up_hi = -1;
   }
-  // hitter grid: a power of two of workgroups (4 waves each) that covers every item the live
+  // hitter grid: a grid class of workgroups (4 waves each) that covers every item the live
   // counts can need, so a few graphs serve any mix of frames
   const int64_t need = (max_hit_items(groups, tail_groups) + 3) / 4;
-  StepGrids grids{1, 1, 1, 1};
-  while (grids.hit_blocks < need) grids.hit_blocks <<= 1;
+  StepGrids grids{grid_class(need), 1, 1, 1};
   if (grids.hit_blocks > (int64_t)1 << 24) return SPLAT_ERR_UNSUPPORTED;  // grid of at most 2^32 threads
   // the striding kernels: powers of two up to 65536 workgroups (a few graph variants in all)
   while (grids.first_blocks < first && grids.first_blocks < 65536) grids.first_blocks <<= 1;
   while (grids.final_blocks < last && grids.final_blocks < 65536) grids.final_blocks <<= 1;
   // one bounce workgroup (and one count wave) per tile of the frames that bounce: tiles <=
   // groups / 4 + slots, so this never passes hit_blocks
-  while (grids.tile_blocks < tiles) grids.tile_blocks <<= 1;
+  grids.tile_blocks = grid_class(tiles);
   if (e->use_graph) {
     hipGraphExec_t& ge = e->graphs[grids.key()];
     if (!ge) {
@@ -750,7 +770,7 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   for (int32_t i = 0; i < e->max_slots; ++i) {
     HostSlot& h = e->slots[i];
     if (!h.used || h.stage < 0) continue;
-    if (h.stage >= h.depth) {
+    if (h.stage >= h.last) {
       p.finishing.push_back(i);
       h.stage = -2;  // finished on the device once this step's event completes
     } else {

@@ -25,12 +25,25 @@ class ResidentScene:
     hitter: api.HIPBVHitter
     name: str = ""
 
+    tuned: dict | None = None  # the autotuner's choice (depth, walk, candidates), when tuned
+
     @staticmethod
-    def build(sc: scene.Scene, depth: int | None = None, device: str = "cuda") -> "ResidentScene":
-        """Uploads the triangles and builds the tree hitter's BVs on the device (0.4-0.8 ms for
-        10k-178k triangles; the host tree_partition takes 30-440 ms)."""
+    def build(sc: scene.Scene, depth: int | None = None, device: str = "cuda", tune: bool = True,
+              tune_cache: str | None = None) -> "ResidentScene":
+        """Uploads the triangles and builds the tree hitter's BVs on the device (0.5-1 ms for
+        10k-178k triangles; the host tree_partition takes 60-1000 ms). With tune (and no fixed
+        depth) the leaf depth and walk mode come from the autotuner, timed on 512^2 frames of a
+        light above the scene (autotune.tune_device_bv; cached per scene and frame shape)."""
+        from . import autotune
+
         tris = api.to_device(sc.ftris, device)
-        return ResidentScene(sc, tris, api.HIPBVHitter.build_on_device(tris, depth), sc.name)
+        if depth is not None or not tune:
+            return ResidentScene(sc, tris, api.HIPBVHitter.build_on_device(tris, depth), sc.name)
+        cam = lights_cam(1, 512, 1, abi.LAMBDA_UNIFORM)
+        p = abi.run_params(4, sc.first_diffuse, math.nan, seed=1)
+        ch = autotune.tune_device_bv(tris, cam, p, abi.lightmap_params(512, 512), cache_path=tune_cache)
+        tuned = {k: v for k, v in ch.items() if k != "hitter"}
+        return ResidentScene(sc, tris, ch["hitter"], sc.name, tuned)
 
 
 @dataclass

@@ -28,3 +28,34 @@ def test_autotune_bf_and_bv(tmp_path):
     assert any(k.startswith("bf:") for k in cached) and any(k.startswith("bv:") for k in cached)
     again = autotune.tune_bv(api.to_device(sc.ftris), sc.ftris, rays, cache_path=path)
     assert (again["depth"], again["group_size"]) == (ch["depth"], ch["group_size"])
+
+
+def test_autotune_device_bv_frames(tmp_path):
+    """The device-built tree hitter tuned on whole frames: every candidate's hit indices agree,
+    the chosen hitter traces the same frame as the untuned one, the choice is cached, and the
+    walk mode survives a rebuild."""
+    import math
+
+    import torch
+
+    from splat_amd import abi
+
+    sc = H.small_c2(3)
+    tris = api.to_device(sc.ftris)
+    cam = scene.camera_c2(seed=9, height=96, width=96)
+    p = abi.run_params(3, sc.first_diffuse, math.nan, seed=5)
+    lm = abi.lightmap_params(64, 64)
+    path = str(tmp_path / "tune.json")
+    ch = autotune.tune_device_bv(tris, cam, p, lm, cache_path=path)
+    assert ch["walk"] in (0, 1, 2) and len(ch["candidates"]) >= 6
+    base = api.Frame(cam, api.HIPBVHitter.build_on_device(tris), tris, p, lm)
+    base.trace()
+    tuned = api.Frame(cam, ch["hitter"], tris, p, lm)
+    tuned.trace()
+    torch.cuda.synchronize()
+    assert torch.equal(base.hit_idx, tuned.hit_idx) and torch.equal(base.rays, tuned.rays)
+    again = autotune.tune_device_bv(tris, cam, p, lm, cache_path=path)
+    assert (again["depth"], again["walk"]) == (ch["depth"], ch["walk"])
+    assert "bvdev" in "".join(json.load(open(path)))
+    ch["hitter"].rebuild(tris)
+    assert ch["hitter"].hs.tune_waves == ch["walk"]

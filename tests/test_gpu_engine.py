@@ -237,3 +237,37 @@ def test_engine_admission_checks_the_light_map(scenes):
         eng.step()
         eng.poll(wait=True)
     eng.close()
+
+
+def test_light_map_only_frames_equal_full_frames(scenes):
+    """Frames that keep only their light map (no final rays or hit indices: plain keep_rays=False,
+    engine requests without outputs) give the light maps of full frames; the engine finishes
+    them with their last bounce, one step before frames that keep their rays."""
+    import torch
+
+    specs = _frames(10, seed=5)
+    for s in specs:
+        s["depth"] = max(s["depth"], 1)
+    frames = [_make(s, scenes) for s in specs]
+    want = [_plain(f) for f in frames]
+    for f, w in zip(frames, want):  # plain, map only
+        fr = api.Frame(f["cam"], f["hitter"], f["tris"], f["p"], f["lm"], keep_rays=False)
+        fr.trace()
+        _sync()
+        assert fr.rays is None and H.rel_l2(fr.tex.cpu().numpy(), w[0]) < 1e-4
+    eng = api.Engine(1 << 30, max_slots=16)
+    for k, f in enumerate(frames):
+        assert eng.admit(cam=f["cam"], hitter=f["hitter"], tris=f["tris"], params=f["p"], lm=f["lm"], tex=f["tex"],
+                         tag=300 + k)
+    done, steps = [], 0
+    while len(done) < len(frames):
+        eng.step()
+        steps += 1
+        done += eng.poll(wait=True)
+        assert steps <= 20
+    assert steps == max(s["depth"] for s in specs)
+    for f, w in zip(frames, want):
+        got = f["tex"].cpu().numpy()
+        assert (np.abs(w[0]).sum() == 0 and np.abs(got).sum() == 0) or H.rel_l2(got, w[0]) < 1e-4
+    eng.close()
+    torch.cuda.empty_cache()
