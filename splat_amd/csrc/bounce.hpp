@@ -75,8 +75,10 @@ __device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in
 
 // The bounce pass's LDS: one instance per workgroup, whichever bounce_tile variant runs. After the
 // tile's records are written out, `stage` holds the privatised light-map bins.
+constexpr int MULTI_BINS = 832;  // bins of the multi-sample splat (its own area: records still staged)
 struct BounceLDS {
   __align__(16) uint2 stage[TILE * WORDS];  // 34,816 bytes: the tile's records
+  float mbins[MULTI_BINS];                  // 3,328 bytes (4 engine workgroups per CU still fit)
   int16_t order[TILE];
   int32_t wave_a[TILE / 64], wave_r[TILE / 64];
   int32_t box[TILE / 64][4];  // per wave: min x, max x, min y, max y of its splat texels
@@ -145,6 +147,102 @@ __device__ __forceinline__ void splat_tile(BounceLDS& lds, bool has, int32_t sx,
   }
 }
 
+// The union over the workgroup of per-wave texel windows [x0, x1] x [y0, y1] (lanes without
+// splats pass an empty window) and the workgroup's count of `n`; false when the union is empty.
+__device__ __forceinline__ bool tile_window(BounceLDS& lds, int32_t x0, int32_t x1, int32_t y0, int32_t y1, int n,
+                                            int32_t& bx0, int32_t& bx1, int32_t& by0, int32_t& by1, int& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  x0 = wave_min_i32(x0); x1 = wave_max_i32(x1); y0 = wave_min_i32(y0); y1 = wave_max_i32(y1);
+  int32_t cnt = n;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  if (lane == 0) {
+    lds.box[wave][0] = x0; lds.box[wave][1] = x1; lds.box[wave][2] = y0; lds.box[wave][3] = y1;
+    lds.wave_r[wave] = cnt;
+  }
+  __syncthreads();
+  bx0 = INT32_MAX; bx1 = INT32_MIN; by0 = INT32_MAX; by1 = INT32_MIN; total = 0;
+  for (int w = 0; w < TILE / 64; ++w) {
+    bx0 = min(bx0, lds.box[w][0]); bx1 = max(bx1, lds.box[w][1]);
+    by0 = min(by0, lds.box[w][2]); by1 = max(by1, lds.box[w][3]);
+    total += lds.wave_r[w];
+  }
+  __syncthreads();  // box and wave_r are reused after this
+  return bx0 <= bx1;
+}
+
+// Every splat sample of the tile's DIFFUSE rays with several samples each (spectral planes,
+// sub-pixel offsets): `diff` marks this lane's final-evolved ray r on triangle F. The texture
+// coordinates of a ray's samples are affine in its expansion offsets (lambda - lambda0, dx, dy),
+// so the samples at the corners of its offset box (at most 8; 2 for a wavelength grid alone)
+// bound the texels of all of them. When the tile's samples crowd the union of those windows (at
+// least two plane writes per bin), they are summed in the LDS bins `mbins` and flushed with one
+// global atomic per touched bin (a sample outside the window, after rounding, goes straight to
+// the map); otherwise every sample goes straight to the map. Called by the whole workgroup.
+__device__ __forceinline__ void splat_tile_multi(BounceLDS& lds, bool diff, const RayF& r, const FTriF& F,
+                                                 const BounceParams& P, const splat_lightmap_params_t& lm) {
+  const int nl = lm.n_lambda > 0 ? lm.n_lambda : 1, nd = lm.n_delta;
+  const int np = splat_planes(lm);
+  const int64_t plane = (int64_t)lm.width * lm.height;
+  int32_t x0 = INT32_MAX, x1 = INT32_MIN, y0 = INT32_MAX, y1 = INT32_MIN;
+  if (diff) {
+    for (int kc = 0; kc < (nl > 1 ? 2 : 1); ++kc)
+      for (int yc = 0; yc < (nd > 1 ? 2 : 1); ++yc)
+        for (int xc = 0; xc < (nd > 1 ? 2 : 1); ++xc) {
+          const int k = kc ? nl - 1 : 0;
+          const float lam = lm.n_lambda > 0 ? lm.spectrum[k] : r.lambda;
+          int32_t tx, ty;
+          float I;
+          if (splat_sample(r, F, lm, lam, xc ? nd - 1 : 0, yc ? nd - 1 : 0, &tx, &ty, &I)) {
+            x0 = min(x0, tx); x1 = max(x1, tx); y0 = min(y0, ty); y1 = max(y1, ty);
+          }
+        }
+  }
+  int32_t bx0, bx1, by0, by1;
+  int n_diff;
+  if (!tile_window(lds, x0, x1, y0, y1, diff ? 1 : 0, bx0, bx1, by0, by1, n_diff)) {
+    // no corner sample landed; any sample that does goes straight to the map (uniform branch)
+    bx0 = 1;
+    bx1 = 0;
+  }
+  const int64_t ww = (int64_t)bx1 - bx0 + 1, wh = (int64_t)by1 - by0 + 1;
+  const int64_t writes = (int64_t)n_diff * nl * nd * nd * np;
+  const int64_t nb = bx0 <= bx1 ? ww * wh * lm.channels : 0;
+  const bool binned = nb > 0 && nb <= MULTI_BINS && writes >= 2 * nb;  // uniform
+  if (binned) {
+    for (int e = threadIdx.x; e < (int)nb; e += TILE) lds.mbins[e] = 0.0f;
+    __syncthreads();
+  }
+  if (diff) {
+    for (int k = 0; k < nl; ++k) {
+      const float lam = lm.n_lambda > 0 ? lm.spectrum[k] : r.lambda;
+      for (int iy = 0; iy < nd; ++iy)
+        for (int ix = 0; ix < nd; ++ix) {
+          int32_t tx, ty;
+          float I;
+          if (!splat_sample(r, F, lm, lam, ix, iy, &tx, &ty, &I)) continue;
+          const bool in = binned && tx >= bx0 && tx <= bx1 && ty >= by0 && ty <= by1;
+          for (int c = 0; c < np; ++c) {
+            const int32_t pl = splat_plane(lm, k, c);
+            const float v = splat_value(lm, k, c, lam, I);
+            if (in) atomicAdd(lds.mbins + (pl * wh + (ty - by0)) * ww + (tx - bx0), v);
+            else unsafeAtomicAdd(P.tex + tx + (int64_t)lm.width * ty + plane * pl, v);
+          }
+        }
+    }
+  }
+  if (binned) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < (int)nb; e += TILE) {
+      const float v = lds.mbins[e];
+      if (v != 0.0f) {
+        const int pl = e / (int)(ww * wh), rem = e - pl * (int)(ww * wh);
+        const int32_t y = by0 + rem / (int)ww, x = bx0 + rem % (int)ww;
+        unsafeAtomicAdd(P.tex + x + (int64_t)lm.width * y + plane * pl, v);
+      }
+    }
+  }
+}
+
 // The bounce pass of one tile (see the top of this file). off_a / off_r: where the tile's ACTIVE
 // and other rays start in out_active / out_rest; hit_rest gets the hit index of the others (either
 // may be null: a frame whose retired rays nobody reads -- only its light map -- skips them);
@@ -177,7 +275,8 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
   // in LDS bins over the texel window they cover, then flushed with one global atomic per
   // touched bin; otherwise every sample goes straight to the map.
   const bool one_sample = P.tex && lm.n_delta == 1 && lm.n_lambda <= 1;
-  bool has = false;  // this ray has a single-sample splat pending
+  bool has = false;   // this ray has a single-sample splat pending
+  bool multi = false;  // this ray is final-evolved in `rec` with several samples pending
   int32_t sx = 0, sy = 0;
   float sv[3] = {0.f, 0.f, 0.f};
   int32_t i = 1;
@@ -203,7 +302,8 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
             for (int c = 0; c < 3; ++c)
               if (c < splat_planes(lm)) sv[c] = splat_value(lm, 0, c, lam, I);
         } else {
-          final_evolve_splat(o, i, F, lm, P.tex);
+          final_evolve(o, i, F);
+          multi = true;
         }
       }
       store_rayf(rec, o);
@@ -235,6 +335,15 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
     if (hit_rest) hit_rest[off_r + pr + rr] = i;
   }
   __syncthreads();
+  if (P.tex && !one_sample) {  // uniform: the final-evolved records are still staged
+    RayF o{};
+    FTriF F{};
+    if (multi) {
+      o = load_rayf(rec);
+      F = load_ftri(P.tris + (i - 1));
+    }
+    splat_tile_multi(lds, multi, o, F, P, lm);
+  }
   write_words(stage, order, 0, na * WORDS, reinterpret_cast<uint2*>(out_active + off_a));
   if (out_rest) write_words(stage, order, na * WORDS, nvalid * WORDS, reinterpret_cast<uint2*>(out_rest + off_r));
   if (one_sample) splat_tile(lds, has, sx, sy, sv, P, lm);

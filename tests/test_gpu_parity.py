@@ -395,6 +395,33 @@ def test_run_evolution_bitexact(hitter_kind, depth, force_rand):
     assert api.from_device(rays_d, abi.ADRAY).tobytes() == want_r.tobytes()
 
 
+@pytest.mark.parametrize("size,mode,spectrum,n_delta", [
+    (8, "spectral", np.arange(400, 701, 25, dtype=np.float32), 1),  # 8x8x13 window: always LDS-binned
+    (16, "spectral", np.array([430.0, 550.0, 660.0], dtype=np.float32), 2),
+    (6, "rgb", None, 3),
+    (200, "spectral", np.arange(400, 701, 50, dtype=np.float32), 2),  # spread out: straight to the map
+])
+def test_fused_multi_sample_splat_vs_oracle(size, mode, spectrum, n_delta):
+    """Frames whose DIFFUSE rays splat several samples each (wavelength planes, sub-pixel
+    offsets), splatted inside the bounce pass: on small maps the tile's samples are summed in LDS
+    bins before the global atomics, on large ones they go straight to the map; either way the map
+    equals the oracle's to 1e-4 and the final rays are bit-exact."""
+    sc = scene.scene_c1()
+    cam = scene.camera_c1(seed=6, height=64, width=64, jitter=0.025)
+    p = abi.run_params(3, sc.first_diffuse, math.nan, seed=31)
+    lm = abi.lightmap_params(size, size, n_delta=n_delta, spectrum=spectrum, mode=mode)
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem, n_top = bv.tree_partition(sc.ftris, 3)
+    fr = api.Frame(cam, api.HIPBVHitter(tris_d, bvs, mem, n_top=n_top), tris_d, p, lm)
+    fr.trace()
+    _sync()
+    r, h, _ = O.run_evolution(O.raygen(cam), sc.ftris, p)
+    want_tex, want_r = O.light_map(r, h, sc.ftris, lm)
+    assert (fr.hit_idx.cpu().numpy() == h).all()
+    assert api.from_device(fr.rays, abi.ADRAY).tobytes() == want_r.tobytes()
+    assert want_tex.sum() > 0 and H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
+
+
 @pytest.mark.parametrize("up,hitter_kind", [(1, "bf"), (2, "bv")])
 def test_trace_lights_vs_oracle(up, hitter_kind):
     """trace!'s forward pass from two RectLights (rays_from_lights -> run_evolution! ->
