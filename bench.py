@@ -231,7 +231,7 @@ def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
                       f"2^{depth} object-median leaves, {params.depth} bounces, splat), OpenMP {threads} threads"}
 
 
-def batched_vs_plain(reps: int = 3):
+def batched_vs_plain(reps: int = 3, tune_cache=None):
     """64 mixed frames (256^2-1024^2 rays, icosphere subdiv 2-5 + plane, 4 bounces): all in one
     engine batch (every frame's work items in shared launches, one graph replay per step) versus the
     same frames traced one frame per launch sequence. Both keep each frame's light map only (the
@@ -240,7 +240,7 @@ def batched_vs_plain(reps: int = 3):
 
     from splat_amd import api, serve
 
-    scenes, reqs = serve.batched_workload(64)
+    scenes, reqs = serve.batched_workload(64, tune_cache=tune_cache)
     rays = sum(r.rays for r in reqs)
     frames = [api.Frame(r.cam, scenes[r.scene].hitter, scenes[r.scene].tris, r.params, r.lm, keep_rays=False)
               for r in reqs]
@@ -285,14 +285,14 @@ def _sweep_point(st, reqs, rate, extra=None):
     return out
 
 
-def served(n_requests: int = 120, rates=(40.0, 150.0, 300.0, 600.0, 1200.0)):
+def served(n_requests: int = 120, rates=(40.0, 150.0, 300.0, 600.0, 1200.0), tune_cache=None):
     """Poisson arrivals over 8 resident scenes (320 to ~178k triangles), resolution uniform over
     256^2-2048^2, 1-16 stratified wavelengths per pixel, 4 bounces, admitted against an HBM budget
     and continuously batched into shared steps; one run per arrival rate, from below to past
     saturation (achieved below offered)."""
     from splat_amd import api, serve
 
-    scenes = serve.serving_scenes()
+    scenes = serve.serving_scenes(tune_cache)
     fd = [s.sc.first_diffuse for s in scenes]
     eng = api.Engine(96 << 30, max_slots=64)
     serve.run_requests(eng, scenes, serve.serving_requests(4, rates[0], len(scenes), fd), realtime=False)  # warm-up
@@ -308,7 +308,7 @@ def served(n_requests: int = 120, rates=(40.0, 150.0, 300.0, 600.0, 1200.0)):
     return out
 
 
-def served_1mp(n_requests: int = 300, rates=(400.0, 1500.0, 2500.0, 3500.0, 5000.0)):
+def served_1mp(n_requests: int = 300, rates=(400.0, 1500.0, 2500.0, 3500.0, 5000.0), tune_cache=None):
     """The latency case: Poisson arrivals of 1 MP requests (1024^2 rays, one seeded wavelength per
     ray, 4 bounces, 1024^2 RGB light map) over the 10,240-triangle two-icosphere scene, each with
     its own pose and light, continuously batched; p50/p99 = completion - arrival. One run per
@@ -319,7 +319,7 @@ def served_1mp(n_requests: int = 300, rates=(400.0, 1500.0, 2500.0, 3500.0, 5000
     from splat_amd import abi, api, scene, serve
 
     sc = scene.scene_c2(seed=2)
-    rs = serve.ResidentScene.build(sc)
+    rs = serve.ResidentScene.build(sc, tune_cache=tune_cache)
 
     def make(rate):
         arr = serve.poisson_arrivals(n_requests, rate, seed=7)
@@ -522,7 +522,7 @@ def main() -> int:
     if args.profile_batched:
         from splat_amd import serve
 
-        scenes, reqs = serve.batched_workload(64)
+        scenes, reqs = serve.batched_workload(64, tune_cache=args.tune_cache)
         arena = int(sum(r.rays for r in reqs) * 440 * 1.05) + (1 << 28)
         eng = api.Engine(arena, max_slots=64)
         walls = []
@@ -563,6 +563,18 @@ def main() -> int:
         lev[i + 1].record()
     torch.cuda.synchronize()
     frame_ms = np.array([lev[i].elapsed_time(lev[i + 1]) for i in range(NL)])
+    # the same frames keeping only their light map (the served product; no final rays or hit
+    # indices, so no retired-ray writes and no realigning hit), timed the same way
+    mfr = api.Frame(cam, hitter, frame.tris, params, lm, keep_rays=False)
+    for _ in range(args.warmup):
+        mfr.trace()
+    torch.cuda.synchronize()
+    t0m = time.perf_counter()
+    for _ in range(K):
+        mfr.trace()
+    torch.cuda.synchronize()
+    wall_map = time.perf_counter() - t0m
+    del mfr
 
     # ---- instrumented frames: hitter and fused bounce-pass launches timed with events recorded
     # on the frame's own stream around each launch ----
@@ -614,11 +626,11 @@ def main() -> int:
 
     del frame
     torch.cuda.empty_cache()
-    batched = None if args.no_batched else batched_vs_plain()
+    batched = None if args.no_batched else batched_vs_plain(tune_cache=args.tune_cache)
     torch.cuda.empty_cache()
-    serve_out = None if args.no_served else served()
+    serve_out = None if args.no_served else served(tune_cache=args.tune_cache)
     torch.cuda.empty_cache()
-    serve_1mp = None if args.no_served else served_1mp()
+    serve_1mp = None if args.no_served else served_1mp(tune_cache=args.tune_cache)
     torch.cuda.empty_cache()
     render = None if args.no_served else render_frame()
     bvb = None if args.no_bv_build else bv_build()
@@ -653,6 +665,8 @@ def main() -> int:
         "p50_ms": round(float(np.percentile(frame_ms, 50)), 4),
         "p99_ms": round(float(np.percentile(frame_ms, 99)), 4),
         "latency_frames": int(frame_ms.size),
+        "value_map_only": round(R * K / wall_map / 1e6, 3),
+        "ms_per_step_map_only": round(wall_map / K * 1e3, 4),
         "roofline": {
             "kernel": "k_hit_bv (bounding-volume hitter)",
             "bound": "valu",  # per-ray FP32 vector ALU work (no matrix instructions); peak: FP32 vector

@@ -489,8 +489,10 @@ class Engine:
         L.check(self._lib.splat_engine_step(self.h, L.stream_handle(stream)), "engine_step")
 
     def poll(self, wait: bool = False, max_tags: int = 4096) -> list[int]:
-        buf = (C.c_int64 * max_tags)()
-        n = C.c_int32()
+        if getattr(self, "_tag_buf", None) is None or len(self._tag_buf) < max_tags:
+            self._tag_buf = (C.c_int64 * max_tags)()  # reused: polled once per step
+            self._n_buf = C.c_int32()
+        buf, n = self._tag_buf, self._n_buf
         L.check(self._lib.splat_engine_poll(self.h, 1 if wait else 0, C.cast(buf, C.c_void_p), max_tags, C.byref(n)),
                 "engine_poll")
         tags = [int(buf[i]) for i in range(n.value)]

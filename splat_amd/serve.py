@@ -149,13 +149,13 @@ def lights_cam(seed: int, res: int, spp: int, lambda_mode: int, scene_extent: fl
                          jitter=0.025, seed=seed)
 
 
-def batched_workload(n_frames: int = 64, seed: int = 4, depth: int = 4):
+def batched_workload(n_frames: int = 64, seed: int = 4, depth: int = 4, tune_cache: str | None = None):
     """64 frames, resolution from {256^2, 512^2, 1024^2}, each over an icosphere of subdivision
     2-5 (320-20,480 glass triangles) above a diffuse plane; pose, light and wavelengths from the
     seed. Returns (scenes, requests)."""
     scs = [scene.assemble([scene.sphere_mesh(k, radius=1.0)], [scene.ground_quad(-1.5, 4.0)], f"icosphere{k}+plane")
            for k in (2, 3, 4, 5)]
-    res_scenes = [ResidentScene.build(sc) for sc in scs]
+    res_scenes = [ResidentScene.build(sc, tune_cache=tune_cache) for sc in scs]
     rng = np.random.default_rng(seed)
     reqs = []
     for k in range(n_frames):
@@ -167,7 +167,7 @@ def batched_workload(n_frames: int = 64, seed: int = 4, depth: int = 4):
     return res_scenes, reqs
 
 
-def serving_scenes():
+def serving_scenes(tune_cache: str | None = None):
     """8 resident scenes from 320 to ~178k triangles."""
     ground = lambda h=4.0: scene.ground_quad(-1.5, h)  # noqa: E731
     scs = [scene.assemble([scene.sphere_mesh(k)], [ground()], f"icosphere{k}+plane") for k in (2, 3, 4, 5, 6)]
@@ -176,7 +176,7 @@ def serving_scenes():
     scs.append(scene.assemble([scene.sphere_mesh(4, center=(0.0, 0.0, 2.0), radius=0.8)],
                               [scene.cube_mesh(cen, edge), ground(5.0)], "menger3+icosphere4+plane"))
     scs.append(scene.scene_c3())
-    return [ResidentScene.build(sc) for sc in scs]
+    return [ResidentScene.build(sc, tune_cache=tune_cache) for sc in scs]
 
 
 def serving_requests(n: int, rate_per_s: float, n_scenes: int, scenes_fd: list[int], seed: int = 5,
