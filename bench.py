@@ -201,6 +201,90 @@ def small_frame(budget_s: float = 3.0):
             "cpu_frames": reps, "gpu_ms_per_frame": round(R / gpu / 1e3, 4), "roofline_bf_hitter": bf}
 
 
+def bf_sweep(reps: int = 5):
+    """The reference's brute-force hitter sweep (src/scenes/perf_test.jl:11-36): R in {256^2, 512^2,
+    768^2, 1024^2} rays from the origin in uniformly random directions (components in [0, 1)),
+    N in {4^0 .. 4^8} random triangles (vertices in [0, 1)^3, after the sentinel); the median of
+    `reps` next_hit! calls, each R x N ray-triangle tests of 40 FLOP."""
+    import numpy as np
+    import torch
+
+    from splat_amd import abi, api
+
+    rng = np.random.default_rng(36)
+    out = []
+    for side in (256, 512, 768, 1024):
+        R = side * side
+        rays = np.zeros(R, dtype=abi.ADRAY)
+        rays["dir"] = rng.uniform(0, 1, (R, 3)).astype(np.float32)
+        rays["ignore_tri"] = 1
+        rays["status"] = abi.RAY_ACTIVE
+        rays_d = api.to_device(rays)
+        tr = api.HIPTracer(rays_d)
+        for e in range(0, 17, 2):
+            N = 1 << e
+            t = np.zeros(N + 1, dtype=abi.NTRI)
+            t["idx"] = np.arange(1, N + 2, dtype=np.int32)
+            v = rng.uniform(0, 1, (N, 3, 3)).astype(np.float32)
+            t["v"][1:, 1:] = v
+            t["v"][1:, 0] = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]).astype(np.float32)
+            hitter = api.HIPBruteForceHitter(api.to_device(t))
+            for _ in range(2):
+                api.next_hit_(tr, hitter, rays_d)
+            torch.cuda.synchronize()
+            ms = []
+            for _ in range(reps):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                api.next_hit_(tr, hitter, rays_d)
+                b.record()
+                torch.cuda.synchronize()
+                ms.append(a.elapsed_time(b))
+            m = float(np.median(ms))
+            tflops = FLOP_PER_TRI_TEST * R * N / (m / 1e3) / 1e12
+            out.append([R, N, round(m, 4), round(tflops, 2), round(tflops / FP32_PEAK_TFLOPS, 4)])
+    return {"columns": ["rays", "triangles", "median_ms", "TFLOP/s", "frac_of_157.3"], "rows": out}
+
+
+def splat_operator(sc, cam, hitter, tris, params, lm, reps: int = 20):
+    """continuum_light_map2! as the standalone operator (splat_light_map: final evolution of the
+    DIFFUSE rays, then the atomic splat) on the 1 MP frame's rays after run_evolution!; bytes per
+    SURVEY's splat formula: R_diffuse x (109 + 4) + touched triangles x 120 + 2 x map bytes."""
+    import numpy as np
+    import torch
+
+    from splat_amd import abi, api
+
+    rays0 = api.rays_from_pinhole(cam)
+    tr = api.HIPTracer(rays0)
+    api.run_evolution_(hitter, tr, depth=params.depth, first_diffuse=params.first_diffuse, tris=tris, rays=rays0,
+                       seed=params.seed, optics=params.optics)
+    torch.cuda.synchronize()
+    host = api.from_device(rays0, abi.ADRAY)
+    dif = host["status"] == abi.RAY_DIFFUSE
+    n_dif = int(dif.sum())
+    touched = int(np.unique(tr.hit_idx.cpu().numpy()[dif]).size)
+    tex = torch.zeros((lm.channels, lm.height, lm.width), dtype=torch.float32, device="cuda")
+    rays = torch.empty_like(rays0)
+    ms = []
+    for k in range(reps + 2):
+        rays.copy_(rays0)
+        tex.zero_()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        api.continuum_light_map2_(tracer=tr, tris=tris, rays=rays, tex=tex, lm=lm)
+        b.record()
+        torch.cuda.synchronize()
+        if k >= 2:
+            ms.append(a.elapsed_time(b))
+    m = float(np.median(ms))
+    nbytes = n_dif * (109 + 4) + touched * 120 + 2 * lm.width * lm.height * lm.channels * 4
+    return {"kernel": "k_light_map (continuum_light_map2!: final evolution + atomic splat, standalone)",
+            "bound": "hbm", "achieved": round(nbytes / (m / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(nbytes / (m / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nbytes, "median_ms": round(m, 4),
+            "diffuse_rays": n_dif, "splats_per_s_G": round(n_dif / (m / 1e3) / 1e9, 2)}
+
+
 def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
     """The oracle's FP32 C++ restatement of the same frame (BV-culled hitter, OpenMP), timed on
     this host on an image-wide strided sample of the frame's rays."""
@@ -456,6 +540,9 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-bv-build", action="store_true")
+    ap.add_argument("--no-bf-sweep", action="store_true")
+    ap.add_argument("--bf-sweep-only", action="store_true",
+                    help="only the brute-force hitter sweep (for a kernel profile); prints its table")
     ap.add_argument("--profile", action="store_true", help="frames only: no instrumentation or CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a 256 MiB device-to-device copy, repeated: calibrates HBM byte counters")
@@ -507,6 +594,9 @@ def main() -> int:
         return 0
     from splat_amd import api
 
+    if args.bf_sweep_only:
+        print(json.dumps(bf_sweep()))
+        return 0
     if args.profile_bf:
         from splat_amd import scene
 
@@ -624,8 +714,10 @@ def main() -> int:
     traffic, traffic_src = hitter_traffic()
     pmc_flop, pmc_src = hitter_pmc_flops()
 
+    splat_op = splat_operator(sc, cam, hitter, frame.tris, params, lm)
     del frame
     torch.cuda.empty_cache()
+    sweep = None if args.no_bf_sweep else bf_sweep()
     batched = None if args.no_batched else batched_vs_plain(tune_cache=args.tune_cache)
     torch.cuda.empty_cache()
     serve_out = None if args.no_served else served(tune_cache=args.tune_cache)
@@ -692,6 +784,7 @@ def main() -> int:
                             "frac": round(pass_bytes / pass_s / 1e9 / HBM_PEAK_GBS, 4), "bytes_per_frame": pass_bytes,
                             "ms_per_frame": round(pass_s * 1e3, 4), "live_before_bounce": live,
                             "diffuse_rays": n_diffuse},
+            "splat_operator": splat_op,
         },
         "hitter": {
             "ms_per_frame": round(hitter_frame_ms, 4),
@@ -707,6 +800,7 @@ def main() -> int:
         "served_1mp": serve_1mp,
         "render": render,
         "bv_build": bvb,
+        "bf_sweep": sweep,
         "large_scene": large,
         "cpu_baseline": cpu,
         "small_frame": small,
