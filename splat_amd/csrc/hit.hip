@@ -284,11 +284,14 @@ void launch_bv_boxes(const splat_ftri_t* tris, const int32_t* members, const int
 }
 
 int choose_bf_waves(int64_t R, int32_t N) {
-  // enough waves to fill 256 CUs x 4 SIMDs several times over, but no more triangle splitting
-  // than the triangle count justifies
-  int64_t groups = (R + 63) / 64;
-  int w = 1;
-  while (w < 16 && groups * w < 8 * 1024 && (N / (2 * w)) >= 32) w *= 2;
+  // Measured on MI355X over rays 256^2..1024^2 x triangles 256..65536: eight waves per group of 64
+  // rays beat one, two and four at every size (55 vs 47 TFLOP/s at 1024^2 rays, 55 vs 38 at
+  // 768^2, where one wave per group leaves a 1.125-round tail on the 8192 wave slots), sixteen
+  // win only when the grid would otherwise hold fewer than two rounds of waves. Each wave keeps
+  // at least 32 triangles.
+  const int64_t groups = (R + 63) / 64;
+  int w = groups * 8 < 2 * 8192 && N >= 1024 ? 16 : 8;
+  while (w > 1 && N / w < 32) w /= 2;
   return w;
 }
 
