@@ -78,12 +78,15 @@ __device__ __forceinline__ int bounce_count(const splat_adray_t* __restrict__ in
 constexpr int MULTI_BINS = 832;  // bins of the multi-sample splat (its own area: records still staged)
 struct BounceLDS {
   __align__(16) uint2 stage[TILE * WORDS];  // 34,816 bytes: the tile's records
-  float mbins[MULTI_BINS];                  // 3,328 bytes (4 engine workgroups per CU still fit)
+  // 3,328 bytes (4 engine workgroups per CU still fit): the multi-sample splat's bins, or, for
+  // single-sample splats weighted by curves, the curves' segment table (CurveTab)
+  float mbins[MULTI_BINS];
   int16_t order[TILE];
   int32_t wave_a[TILE / 64], wave_r[TILE / 64];
   int32_t box[TILE / 64][4];  // per wave: min x, max x, min y, max y of its splat texels
 };
 constexpr int SPLAT_BINS = TILE * WORDS * 2;  // floats of `stage`: 8,704
+static_assert(sizeof(CurveTab) <= MULTI_BINS * sizeof(float), "the curve table shares the multi-sample bins");
 
 // max/min over the wave (all 64 lanes take part)
 __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
@@ -264,6 +267,14 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool valid = tid < nvalid;
   splat_adray_t* rec = reinterpret_cast<splat_adray_t*>(stage) + tid;
+  // One splat sample per ray (one wavelength, no sub-pixel offsets): the tile's splats are summed
+  // in LDS bins over the texel window they cover, then flushed with one global atomic per
+  // touched bin; otherwise every sample goes straight to the map.
+  const bool one_sample = P.tex && lm.n_delta == 1 && lm.n_lambda <= 1;
+  // single-sample curve weights read the curves' segment table (in the multi-sample bins' area)
+  const bool curve_tab = one_sample && lm.weight_mode == SPLAT_WEIGHTS_CURVES;
+  CurveTab& ctab = *reinterpret_cast<CurveTab*>(lds.mbins);
+  if (curve_tab) curve_tab_fill(ctab, lm, tid, TILE);
   if constexpr (CAM) {
     if (valid) *rec = raygen_ray(cam, base + tid);
   } else {
@@ -271,10 +282,6 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
   }
   __syncthreads();
 
-  // One splat sample per ray (one wavelength, no sub-pixel offsets): the tile's splats are summed
-  // in LDS bins over the texel window they cover, then flushed with one global atomic per
-  // touched bin; otherwise every sample goes straight to the map.
-  const bool one_sample = P.tex && lm.n_delta == 1 && lm.n_lambda <= 1;
   bool has = false;   // this ray has a single-sample splat pending
   bool multi = false;  // this ray is final-evolved in `rec` with several samples pending
   int32_t sx = 0, sy = 0;
@@ -300,7 +307,7 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
           if (has)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-              if (c < splat_planes(lm)) sv[c] = splat_value(lm, 0, c, lam, I);
+              if (c < splat_planes(lm)) sv[c] = curve_tab ? I * curve_eval_tab(ctab, lm, c, lam) : splat_value(lm, 0, c, lam, I);
         } else {
           final_evolve(o, i, F);
           multi = true;

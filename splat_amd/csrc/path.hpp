@@ -184,6 +184,38 @@ __device__ __forceinline__ float curve_eval(const splat_lightmap_params_t& lm, i
   return FMA(slope, lam - x1, y1);
 }
 
+// The curves' segments as a table a workgroup fills once: for curve c and segment m, its right
+// end (x1, y1) and slope (y1 - y0) / (x1 - x0), the same FP32 operations curve_eval performs, so
+// curve_eval_tab returns the same value. Finding the segment then takes one compare and one
+// select per breakpoint (the abscissae stay wave-uniform operands) instead of carrying four
+// values through every breakpoint.
+struct CurveTab {
+  float x1[3][SPLAT_MAX_CURVE - 1], y1[3][SPLAT_MAX_CURVE - 1], slope[3][SPLAT_MAX_CURVE - 1];
+};
+// thread tid of nthreads fills its share of T (the caller synchronises before any read)
+__device__ __forceinline__ void curve_tab_fill(CurveTab& T, const splat_lightmap_params_t& lm, int tid, int nthreads) {
+  for (int e = tid; e < 3 * (SPLAT_MAX_CURVE - 1); e += nthreads) {
+    const int c = e / (SPLAT_MAX_CURVE - 1), m = e - c * (SPLAT_MAX_CURVE - 1);
+    if (m + 1 < lm.curve_n[c]) {
+      const float x0 = lm.curve_x[c][m], x1 = lm.curve_x[c][m + 1], y0 = lm.curve_y[c][m], y1 = lm.curve_y[c][m + 1];
+      T.x1[c][m] = x1;
+      T.y1[c][m] = y1;
+      T.slope[c][m] = (y1 - y0) / (x1 - x0);
+    }
+  }
+}
+// curve_eval through the table: segment m = the last with lam >= x[m] (x ascending, so the same
+// segment as curve_eval's scan; 0 when lam is NaN or below x[1])
+__device__ __forceinline__ float curve_eval_tab(const CurveTab& T, const splat_lightmap_params_t& lm, int c, float lam) {
+  const int n = lm.curve_n[c];
+  int m = 0;
+  // unrolled over the largest curve, so the abscissae load together (a loop over n would wait
+  // for each one's scalar load in turn)
+#pragma unroll
+  for (int j = 1; j < SPLAT_MAX_CURVE - 1; ++j) m = j < n - 1 && lam >= lm.curve_x[c][j] ? j : m;
+  return FMA(T.slope[c][m], lam - T.x1[c][m], T.y1[c][m]);
+}
+
 // final_evolution (src/ray_imagers.jl:4-37) of one DIFFUSE ray, in place, given its hit test on
 // triangle i (hit, dual distance tD)
 __device__ __forceinline__ void final_evolve_hit(RayF& r, int32_t i, bool hit, const D3<float>& tD) {

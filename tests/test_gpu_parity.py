@@ -511,6 +511,38 @@ def test_trace_frame_vs_oracle():
     assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
 
 
+@pytest.mark.parametrize("lam", [None, 534.0, 399.0, 350.0, 750.0, 612.5])
+def test_fused_splat_curve_weights_exact_on_single_splat_texels(lam):
+    """The fused bounce pass weights a single-sample splat by the retina curves through a
+    per-workgroup segment table. Every texel that exactly one splat reaches (a single float atomic
+    onto zero, so no summation order) must equal the oracle's bit for bit: own wavelengths drawn
+    uniformly, and every ray at one wavelength -- on a breakpoint shared by two curves (534), on
+    the first abscissa (399), below and above the curves (the end segments extended) and
+    mid-segment. The rest of the map within the usual tolerance."""
+    sc = H.small_c2(2)
+    kw = {} if lam is None else dict(lambda_mode=abi.LAMBDA_FIXED, lambda_min=lam, lambda_max=lam)
+    cam = scene.pinhole((0.2, 0.3, 6.0), (0.0, 0.0, -1.5), (0.0, 1.0, 0.0), 90.0, 32, 32, jitter=0.025, seed=12, **kw)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=12)
+    lm = abi.lightmap_params(384, 384)
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem = bv.bv_partition(sc.ftris, 4)
+    fr = api.Frame(cam, api.HIPBVHitter(tris_d, bvs, mem), tris_d, p, lm)
+    fr.trace()
+    _sync()
+    got = fr.tex.cpu().numpy()
+    want_r, want_h, _ = O.run_evolution(O.raygen(cam), sc.ftris, p)
+    want_tex, _ = O.light_map(want_r, want_h, sc.ftris, lm)
+    assert H.rel_l2(got, want_tex) < 1e-4
+    # texels each diffuse ray reaches, one oracle splat at a time
+    count = np.zeros(want_tex.shape[1:], dtype=np.int32)
+    for r in np.flatnonzero(want_r["status"] == abi.RAY_DIFFUSE):
+        one, _ = O.light_map(want_r[r:r + 1], want_h[r:r + 1], sc.ftris, lm)
+        count += (one != 0).any(axis=0)
+    single = count == 1
+    assert single.sum() > 200
+    assert np.array_equal(got[:, single], want_tex[:, single])
+
+
 def test_frame_where_every_ray_finishes_at_the_first_bounce():
     """Only diffuse triangles: after the first bounce no ray stays ACTIVE, so the later bounces
     run with no live ray (the bounce pass, which sums the tile counts itself after the first
