@@ -295,12 +295,14 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
       placed = predict_status(i, P.first_diffuse, P.tris, ray.pos, ray.dir, ray.ignore_tri);
       const FTriF F = load_ftri(P.tris + (i - 1));
       const float u = bounce_random(P.force_rand, nullptr, 0, P.seed, P.bounce, ray.dest);
-      // (final_evolution repeats evolve's hit test rather than keeping its dual distance live
-      // across evolve: measured faster, the registers are the scarce resource here)
-      RayF o = evolve_one(ray, i, F, u, P.first_diffuse, P.optics);
+      // a ray that becomes DIFFUSE is final-evolved with evolve's own hit test (same ray, same
+      // triangle)
+      bool dhit = false;
+      D3<float> dtD;
+      RayF o = evolve_one(ray, i, F, u, P.first_diffuse, P.optics, &dhit, &dtD);
       if (P.tex && o.status == SPLAT_RAY_DIFFUSE) {
         if (one_sample) {
-          final_evolve(o, i, F);
+          final_evolve_hit(o, i, dhit, dtD);
           const float lam = lm.n_lambda > 0 ? lm.spectrum[0] : o.lambda;
           float I;
           has = splat_sample(o, F, lm, lam, 0, 0, &sx, &sy, &I);
@@ -309,7 +311,7 @@ __device__ __forceinline__ void bounce_tile(const splat_adray_t* __restrict__ in
             for (int c = 0; c < 3; ++c)
               if (c < splat_planes(lm)) sv[c] = curve_tab ? I * curve_eval_tab(ctab, lm, c, lam) : splat_value(lm, 0, c, lam, I);
         } else {
-          final_evolve(o, i, F);
+          final_evolve_hit(o, i, dhit, dtD);
           multi = true;
         }
       }

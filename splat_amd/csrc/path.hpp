@@ -99,14 +99,24 @@ __device__ __forceinline__ uint8_t predict_status(int32_t i, int32_t first_diffu
   return status_after(i, first_diffuse, need && sentinel_hit(tris, pos, dir, ignore_tri));
 }
 
-// evolve_ray + handle_optics (src/tracer.jl:181-201, :118-179)
+// evolve_ray + handle_optics (src/tracer.jl:181-201, :118-179). A ray that becomes DIFFUSE keeps
+// its position and direction, so its hit test (hit, dual distance) is what final_evolution's own
+// test on the same triangle gives: returned in *diffuse_hit / *diffuse_tD when asked for.
 __device__ __forceinline__ RayF evolve_one(const RayF& r, int32_t i, const FTriF& F, float rndm, int32_t first_diffuse,
-                                           const splat_optics_t& opt) {
+                                           const splat_optics_t& opt, bool* diffuse_hit = nullptr,
+                                           D3<float>* diffuse_tD = nullptr) {
   D3<float> tD;
   bool hit = get_hit_ad(r, F, i, &tD);
   int32_t i2 = hit ? i : 1;
   RayF out = r;
-  if (i2 >= first_diffuse) { out.status = SPLAT_RAY_DIFFUSE; return out; }
+  if (i2 >= first_diffuse) {
+    out.status = SPLAT_RAY_DIFFUSE;
+    if (diffuse_hit) {
+      *diffuse_hit = hit;
+      *diffuse_tD = tD;
+    }
+    return out;
+  }
   if (!hit) { out.status = SPLAT_RAY_INFINITY; return out; }
 
   D3<float> g = glass_ior(opt, r.lambda), a{1.0f, {0.0f, 0.0f, 0.0f}};
