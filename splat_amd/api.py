@@ -478,7 +478,8 @@ class Engine:
         rq.hit_out = None if hit_out is None else hit_out.data_ptr()
         rq.tag = int(tag)
         rq.lm = lm
-        code = self._lib.splat_engine_admit(self.h, L.ref(rq), L.stream_handle(stream))
+        # admission enqueues nothing (the next step uploads it), so no stream is looked up here
+        code = self._lib.splat_engine_admit(self.h, L.ref(rq), None if stream is None else L.stream_handle(stream))
         if code == -6:
             return False
         L.check(code, "engine_admit")

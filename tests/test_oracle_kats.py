@@ -207,9 +207,12 @@ def test_refract_closed_form(theta_deg, n1, n2):
     z = 0 plane, normalize(eta sin t, 0, -2 eta cos t - sqrt(1 - eta^2 sin^2 t))."""
     th = math.radians(theta_deg)
     eta = n1 / n2
-    if eta * math.sin(th) >= 1:
-        pytest.skip("total internal reflection")
     v = (math.sin(th), 0.0, -math.cos(th))
+    if eta * math.sin(th) >= 1:
+        # total internal reflection: the formula's square root of a negative number (callers
+        # test can_refract first, src/tracer.jl:120)
+        assert np.isnan(_refract(v, (0, 0, 1), n1, n2)).all()
+        return
     want = np.array([eta * math.sin(th), 0.0, -2 * eta * math.cos(th) - math.sqrt(1 - (eta * math.sin(th)) ** 2)])
     want /= np.linalg.norm(want)
     assert np.allclose(_refract(v, (0, 0, 1), n1, n2), want, atol=1e-14)
