@@ -511,6 +511,28 @@ def test_trace_frame_vs_oracle():
     assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
 
 
+@pytest.mark.parametrize("height", [768, 760])
+def test_large_frame_vs_oracle(height):
+    """1024 x 768 rays are 12,288 64-ray groups: the first bounce's hitter runs one item per group
+    and stores its keys, so the frame start writes no miss keys first. 1024 x 760 stays below that
+    (miss keys written, items merged). Both equal the oracle: hit indices and final rays bit for
+    bit, the light map within 1e-4."""
+    sc = H.small_c2(2)
+    cam = scene.camera_c2(seed=9, height=height, width=1024, jitter=0.025)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=9)
+    lm = abi.lightmap_params(256, 256)
+    tris_d = api.to_device(sc.ftris)
+    bvs, mem = bv.bv_partition(sc.ftris, 4)
+    fr = api.Frame(cam, api.HIPBVHitter(tris_d, bvs, mem), tris_d, p, lm)
+    fr.trace()
+    _sync()
+    want_r, want_h, _ = O.run_evolution(O.raygen(cam), sc.ftris, p)
+    want_tex, want_r2 = O.light_map(want_r, want_h, sc.ftris, lm)
+    assert (fr.hit_idx.cpu().numpy() == want_h).all()
+    assert api.from_device(fr.rays, abi.ADRAY).tobytes() == want_r2.tobytes()
+    assert H.rel_l2(fr.tex.cpu().numpy(), want_tex) < 1e-4
+
+
 @pytest.mark.parametrize("lam", [None, 534.0, 399.0, 350.0, 750.0, 612.5])
 def test_fused_splat_curve_weights_exact_on_single_splat_texels(lam):
     """The fused bounce pass weights a single-sample splat by the retina curves through a
