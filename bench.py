@@ -248,8 +248,9 @@ def bf_sweep(reps: int = 5):
 
 def splat_operator(sc, cam, hitter, tris, params, lm, reps: int = 20):
     """continuum_light_map2! as the standalone operator (splat_light_map: final evolution of the
-    DIFFUSE rays, then the atomic splat) on the 1 MP frame's rays after run_evolution!; bytes per
-    SURVEY's splat formula: R_diffuse x (109 + 4) + touched triangles x 120 + 2 x map bytes."""
+    DIFFUSE rays, then the atomic splat) on the 1 MP frame's rays after run_evolution!; algorithmic
+    bytes: per diffuse ray its record's 109 used bytes and its 4-byte hit index, each touched
+    triangle's 120 bytes once, the map read and written once."""
     import numpy as np
     import torch
 
@@ -283,6 +284,47 @@ def splat_operator(sc, cam, hitter, tris, params, lm, reps: int = 20):
             "bound": "hbm", "achieved": round(nbytes / (m / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(nbytes / (m / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nbytes, "median_ms": round(m, 4),
             "diffuse_rays": n_dif, "splats_per_s_G": round(n_dif / (m / 1e3) / 1e9, 2)}
+
+
+def partition_operator(cam, hitter, tris, params, reps: int = 20):
+    """partition! as the standalone operator (splat_partition_active: per-tile ACTIVE counts, scan,
+    stable scatter into the swap buffer, copy back) on the 1 MP frame's rays after the first
+    bounce's evolve (ACTIVE, DIFFUSE and escaped rays mixed); algorithmic bytes: each 136-byte
+    record read and written once plus its status byte, R x (2 x 136 + 1). partition! also copies
+    the result back into `vals` (src/partition.jl:124-131 does the same), another read and write of
+    every record: `with_copy_back` counts those bytes too."""
+    import numpy as np
+    import torch
+
+    from splat_amd import abi, api
+
+    rays0 = api.rays_from_pinhole(cam)
+    tr = api.HIPTracer(rays0)
+    api.next_hit_(tr, hitter, rays0)
+    api.evolve_(rays0, tr.hit_idx, tris, first_diffuse=params.first_diffuse, seed=params.seed, bounce=1,
+                optics=params.optics)
+    torch.cuda.synchronize()
+    R = int(rays0.shape[0])
+    active = int((api.from_device(rays0, abi.ADRAY)["status"] == abi.RAY_ACTIVE).sum())
+    rays = torch.empty_like(rays0)
+    ms = []
+    for k in range(reps + 2):
+        rays.copy_(rays0)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        api.partition_(rays, tr.ray_swap, tr, host_count=False)
+        b.record()
+        torch.cuda.synchronize()
+        if k >= 2:
+            ms.append(a.elapsed_time(b))
+    m = float(np.median(ms))
+    nbytes = R * (2 * abi.ADRAY.itemsize + 1)
+    return {"kernel": "k_count_active + k_scan_tiles + k_scatter (partition!, standalone)", "bound": "hbm",
+            "achieved": round(nbytes / (m / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(nbytes / (m / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nbytes, "median_ms": round(m, 4),
+            "rays": R, "active": active,
+            "with_copy_back": {"bytes": 2 * nbytes - R, "achieved": round((2 * nbytes - R) / (m / 1e3) / 1e9, 1),
+                               "frac": round((2 * nbytes - R) / (m / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
@@ -715,6 +757,7 @@ def main() -> int:
     pmc_flop, pmc_src = hitter_pmc_flops()
 
     splat_op = splat_operator(sc, cam, hitter, frame.tris, params, lm)
+    part_op = partition_operator(cam, hitter, frame.tris, params)
     del frame
     torch.cuda.empty_cache()
     sweep = None if args.no_bf_sweep else bf_sweep()
@@ -785,6 +828,7 @@ def main() -> int:
                             "ms_per_frame": round(pass_s * 1e3, 4), "live_before_bounce": live,
                             "diffuse_rays": n_diffuse},
             "splat_operator": splat_op,
+            "partition_operator": part_op,
         },
         "hitter": {
             "ms_per_frame": round(hitter_frame_ms, 4),
