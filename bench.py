@@ -134,7 +134,16 @@ def host_info() -> dict:
     return {"cpu_model": model, "nproc": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
-def small_frame(budget_s: float = 3.0):
+def bf_frame_hitter(tris, cam, tune_cache=None):
+    """The brute-force hitter of the 320-triangle frame, with the waves per 64-ray group the
+    autotuner picks on the frame's first-bounce rays (cached per ray and triangle count)."""
+    from splat_amd import api, autotune
+
+    choice = autotune.tune_bf(tris, api.rays_from_pinhole(cam), cache_path=tune_cache)
+    return api.HIPBruteForceHitter(tris, tune_waves=choice["waves"]), choice
+
+
+def small_frame(budget_s: float = 3.0, tune_cache=None):
     """The 320-triangle workload (icosphere subdivision 2 over a plane, 256x256 rays, one seeded
     wavelength each, brute-force hitter, 2 bounces, 512x512 RGB map): GPU plain-path rate and the
     oracle's CPU rate on whole frames."""
@@ -149,7 +158,7 @@ def small_frame(budget_s: float = 3.0):
     tris = api.to_device(sc.ftris)
     p = abi.run_params(2, sc.first_diffuse, math.nan, seed=2024)
     lm = abi.lightmap_params(512, 512)
-    hitter = api.HIPBruteForceHitter(tris)
+    hitter, tuned = bf_frame_hitter(tris, cam, tune_cache)
     fr = api.Frame(cam, hitter, tris, p, lm)
     R = fr.R
     for _ in range(10):
@@ -198,7 +207,8 @@ def small_frame(budget_s: float = 3.0):
     cpu = n / (time.perf_counter() - t) / 1e6
     return {"workload": "320-triangle icosphere + plane, 256x256 rays, brute force, 2 bounces, 512x512x3 map",
             "gpu_mrays_per_s": round(gpu, 2), "cpu_mrays_per_s": round(cpu, 3), "cpu_threads": threads,
-            "cpu_frames": reps, "gpu_ms_per_frame": round(R / gpu / 1e3, 4), "roofline_bf_hitter": bf}
+            "cpu_frames": reps, "gpu_ms_per_frame": round(R / gpu / 1e3, 4), "roofline_bf_hitter": bf,
+            "autotuned": {"waves": tuned["waves"]}}
 
 
 def bf_sweep(reps: int = 5):
@@ -644,7 +654,8 @@ def main() -> int:
 
         sc = scene.scene_c1()
         tris = api.to_device(sc.ftris)
-        fr = api.Frame(scene.camera_c1(seed=1), api.HIPBruteForceHitter(tris), tris,
+        cam1 = scene.camera_c1(seed=1)
+        fr = api.Frame(cam1, bf_frame_hitter(tris, cam1, args.tune_cache)[0], tris,
                        abi.run_params(2, sc.first_diffuse, math.nan, seed=2024), abi.lightmap_params(512, 512))
         for _ in range(max(args.steps, 1)):
             fr.trace()
@@ -771,7 +782,7 @@ def main() -> int:
     bvb = None if args.no_bv_build else bv_build()
     large = None if args.no_served else large_frame()
     cpu = None if args.no_cpu_baseline else cpu_baseline(sc, cam, params, lm)
-    small = None if args.no_cpu_baseline else small_frame()
+    small = None if args.no_cpu_baseline else small_frame(tune_cache=args.tune_cache)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
