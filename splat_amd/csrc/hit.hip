@@ -216,7 +216,9 @@ __global__ void k_group_boxes(const float4* __restrict__ boxes, int32_t n_bv, in
   groups[2 * g + 1] = hi;
 }
 
-// tree leaves: node 2^L + b <- box of BV b (empty when b is padding or has no members)
+// tree leaves: node 2^L + b <- box of BV b (empty when b is padding or has no members), with its
+// member range: count in the code's upper bits, first record in hi.w (a walk that reaches the
+// leaf has its range with the box, no dependent load)
 __global__ void k_tree_leaves(const float4* __restrict__ boxes, const int2* __restrict__ ranges, int32_t n_bv,
                               int32_t levels, float4* __restrict__ nodes) {
   const int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -224,14 +226,15 @@ __global__ void k_tree_leaves(const float4* __restrict__ boxes, const int2* __re
   if (b >= n_leaves) return;
   const int32_t n = n_leaves + b;
   float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-  int32_t code = TREE_EMPTY;
+  int32_t code = TREE_EMPTY, first = 0;
   if (b < n_bv && ranges[b].y > 0) {
     lo = boxes[2 * b];
     hi = boxes[2 * b + 1];
-    code = 0;
+    code = ranges[b].y << TREE_COUNT_SHIFT;
+    first = ranges[b].x;
   }
   lo.w = __int_as_float(code);
-  hi.w = 0.f;
+  hi.w = __int_as_float(first);
   nodes[2 * n] = lo;
   nodes[2 * n + 1] = hi;
 }

@@ -271,9 +271,8 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       bi = key_index(k);
     }
   };
-  // every lane tests the members of leaf b (some lane entered it)
-  auto scan_members = [&](int32_t b, bool entered) {
-    const int2 rg = ranges[b];
+  // every lane tests the member records rg = (first, count) of a leaf (some lane entered it)
+  auto scan_range = [&](int2 rg, bool entered) {
     if constexpr (STATS) {
       n_enter += entered ? 1ull : 0ull;
       n_alg += entered ? (unsigned long long)rg.y : 0ull;
@@ -282,6 +281,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
     for (int32_t m = 0; m < rg.y; m += 16)
       scan_records16(S.recs + rg.x + m, rg.y - m < 16 ? rg.y - m : 16, L, best, bi);
   };
+  auto scan_members = [&](int32_t b, bool entered) { scan_range(ranges[b], entered); };
 
   // the records every ray tests before the structure (large triangles): in one split only, the
   // splits' keys merge into the minimum over all of them
@@ -373,15 +373,20 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
       // the child of a node with split code c that the wave's first live ray meets first (0 or 1):
       // the lower one along the split axis when the ray points up it
       auto near_half = [&](int32_t c) { return ((neg >> (c & TREE_AXIS)) & 1) ^ ((c & TREE_SWAP) ? 1 : 0); };
-      auto leaf = [&](int32_t m, bool e) {
+      // a leaf node's member range, stored with its box
+      auto leaf_range = [](const float4& lo, const float4& hi) {
+        return make_int2(__float_as_int(hi.w), __float_as_int(lo.w) >> TREE_COUNT_SHIFT);
+      };
+      auto leaf = [&](int2 rg, bool e) {
         const uint64_t k = fetch();
-        scan_members(m - leaf0, e);
+        scan_range(rg, e);
         merge(k);
       };
-      // leaf m whose box this lane entered at distance t, unless the hits found since rule it out
-      auto visit = [&](int32_t m, float t) {
+      // the leaf of range rg whose box this lane entered at distance t, unless the hits found
+      // since rule it out
+      auto visit = [&](int2 rg, float t) {
         const bool e = t <= best;
-        if (__builtin_amdgcn_ballot_w64(e) != 0) leaf(m, e);
+        if (__builtin_amdgcn_ballot_w64(e) != 0) leaf(rg, e);
       };
       {
         const float t = lane_t(nodes[2 * root], nodes[2 * root + 1]);
@@ -391,7 +396,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
         }
         const uint64_t m = __builtin_amdgcn_ballot_w64(t <= best);
         if (m != 0) {
-          if (root >= leaf0) visit(root, t);
+          if (root >= leaf0) visit(leaf_range(nodes[2 * root], nodes[2 * root + 1]), t);
           else push(root, m);
         }
       }
@@ -413,8 +418,9 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
           }
           const bool wa = __builtin_amdgcn_ballot_w64(ta <= best) != 0, wb = __builtin_amdgcn_ballot_w64(tb <= best) != 0;
           const bool b_first = wb && (!wa || near_half(__float_as_int(nodes[2 * n].w)) == 1);
-          visit(2 * n + (b_first ? 1 : 0), b_first ? tb : ta);
-          if (wa && wb) visit(2 * n + (b_first ? 0 : 1), b_first ? ta : tb);
+          const int2 ra = leaf_range(nodes[4 * n], nodes[4 * n + 1]), rb = leaf_range(nodes[4 * n + 2], nodes[4 * n + 3]);
+          visit(b_first ? rb : ra, b_first ? tb : ta);
+          if (wa && wb) visit(b_first ? ra : rb, b_first ? ta : tb);
           continue;
         }
         // named values, no arrays: a select between array elements puts the array in scratch
@@ -457,10 +463,13 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
           auto pick = [](int32_t j, float a0, float a1, float a2, float a3) {
             return j == 0 ? a0 : j == 1 ? a1 : j == 2 ? a2 : a3;
           };
-          if ((wmask >> i0) & 1u) visit(4 * n + i0, pick(i0, t0, t1, t2, t3));
-          if ((wmask >> i1) & 1u) visit(4 * n + i1, pick(i1, t0, t1, t2, t3));
-          if ((wmask >> i2) & 1u) visit(4 * n + i2, pick(i2, t0, t1, t2, t3));
-          if ((wmask >> i3) & 1u) visit(4 * n + i3, pick(i3, t0, t1, t2, t3));
+          const int2 r0 = leaf_range(nodes[g], nodes[g + 1]), r1 = leaf_range(nodes[g + 2], nodes[g + 3]),
+                     r2 = leaf_range(nodes[g + 4], nodes[g + 5]), r3 = leaf_range(nodes[g + 6], nodes[g + 7]);
+          auto pick_r = [&](int32_t j) { return j == 0 ? r0 : j == 1 ? r1 : j == 2 ? r2 : r3; };
+          if ((wmask >> i0) & 1u) visit(pick_r(i0), pick(i0, t0, t1, t2, t3));
+          if ((wmask >> i1) & 1u) visit(pick_r(i1), pick(i1, t0, t1, t2, t3));
+          if ((wmask >> i2) & 1u) visit(pick_r(i2), pick(i2, t0, t1, t2, t3));
+          if ((wmask >> i3) & 1u) visit(pick_r(i3), pick(i3, t0, t1, t2, t3));
         } else {
           auto mask_of = [&](int32_t j) { return j == 0 ? m0 : j == 1 ? m1 : j == 2 ? m2 : m3; };
           if ((wmask >> i3) & 1u) push(4 * n + i3, mask_of(i3));

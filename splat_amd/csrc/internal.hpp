@@ -25,6 +25,9 @@ void launch_group_boxes(const float4* boxes, int32_t n_bv, int32_t group_size, f
 // the node's box; nodes[2n].w holds a code: bits 0-1 = the axis that separates the children's
 // centres most, bit 2 = the right child lies lower on it, bit 3 = empty subtree.
 constexpr int32_t TREE_AXIS = 3, TREE_SWAP = 4, TREE_EMPTY = 8;
+// a non-empty leaf node carries its member range: count = code >> TREE_COUNT_SHIFT, first record
+// index = the bits of nodes[2n + 1].w
+constexpr int32_t TREE_COUNT_SHIFT = 4;
 constexpr int32_t MAX_TREE_LEVELS = 24;
 // light-map parameters a kernel can use (sizes in range; curve abscissae ascending)
 int32_t check_lightmap(const splat_lightmap_params_t* lm);
