@@ -448,13 +448,14 @@ class Engine:
     frames of different scenes, sizes and depths share every launch; a step advances each
     admitted frame one stage and replays one captured HIP graph."""
 
-    def __init__(self, arena_bytes: int, max_slots: int = 256, use_graph: bool = True):
+    def __init__(self, arena_bytes: int, max_slots: int = 256, use_graph: bool = True, use_queue: bool = True):
         L.require_gpu()
         self._lib = L.lib()
         h = C.c_void_p()
         L.check(self._lib.splat_engine_create(int(arena_bytes), int(max_slots), C.byref(h)), "engine_create")
         self.h = h
         L.check(self._lib.splat_engine_use_graph(self.h, 1 if use_graph else 0), "engine_use_graph")
+        L.check(self._lib.splat_engine_use_queue(self.h, 1 if use_queue else 0), "engine_use_queue")
         self._keep = {}  # tag -> objects that must outlive the frame
 
     def frame_bytes(self, cam: abi.Pinhole, with_outputs: bool = False) -> int:

@@ -82,6 +82,7 @@ SIGNATURES = {
     "splat_engine_sync_steps": (i32, [P, i32]),
     "splat_engine_arena_free": (i64, [P]),
     "splat_engine_use_graph": (i32, [P, i32]),
+    "splat_engine_use_queue": (i32, [P, i32]),
 }
 
 

@@ -85,12 +85,14 @@ def _admit(eng, f, tag):
                      tag=tag, rays_out=f["rays"], hit_out=f["hit"])
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_engine_batch_equals_plain(scenes, use_graph):
+@pytest.mark.parametrize("use_graph,use_queue", [(True, True), (False, True), (True, False)])
+def test_engine_batch_equals_plain(scenes, use_graph, use_queue):
+    """Graph-replayed or stream-launched steps; the hitter as one persistent launch taking its items
+    from the device work queues, or one wave per item over a host-sized grid."""
     specs = _frames(12, seed=1)
     frames = [_make(s, scenes) for s in specs]
     want = [_plain(f) for f in frames]
-    eng = api.Engine(1 << 30, max_slots=16, use_graph=use_graph)
+    eng = api.Engine(1 << 30, max_slots=16, use_graph=use_graph, use_queue=use_queue)
     for k, f in enumerate(frames):
         assert _admit(eng, f, 100 + k)
     done, steps = [], 0
@@ -128,11 +130,13 @@ def test_engine_many_slots(scenes):
     eng.close()
 
 
-def test_engine_many_flat_tails_beside_a_split_walk(scenes):
+@pytest.mark.parametrize("use_queue", [True, False])
+def test_engine_many_flat_tails_beside_a_split_walk(scenes, use_queue):
     """188 frames of 64 ray groups each (each small enough to scan the leaves flat) in one step with
     a 700-group frame whose tree walk is split 32 ways: 22,400 walk items plus 12,032 flat items,
-    more than the hitter grid held before the flat-item bound counted the tail frames' groups.
-    Every frame still equals its single-frame trace."""
+    more than the hitter grid held before the flat-item bound counted the tail frames' groups (and
+    more than the queue hitter's waves: each takes several). Every frame still equals its
+    single-frame trace."""
     rng = np.random.default_rng(11)
     specs = [dict(si=1, h=224, w=200, depth=3, fr=math.nan, seed=901, mode="rgb", nd=1, lmw=96, lmh=80)]
     for k in range(188):
@@ -140,7 +144,7 @@ def test_engine_many_flat_tails_beside_a_split_walk(scenes):
                           seed=int(rng.integers(1, 1 << 30)), mode="mono", nd=1, lmw=48, lmh=40))
     frames = [_make(s, scenes) for s in specs]
     want = [_plain(f) for f in frames]
-    eng = api.Engine(1 << 30, max_slots=256)
+    eng = api.Engine(1 << 30, max_slots=256, use_queue=use_queue)
     for k, f in enumerate(frames):
         assert _admit(eng, f, 700 + k)
     done, steps = [], 0
