@@ -30,6 +30,9 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_TRI_TEST = 40  # distance_to_plane + in_triangle + p = o + t d, per-triangle terms hoisted
 FLOP_PER_BOX_TEST = 12  # slab test with precomputed reciprocals
+# VALU issue peak: each of a CU's 4 SIMDs issues one wave64 vector instruction every 4 cycles (16
+# lanes a cycle); MI355X's published peak engine clock
+PEAK_CLOCK_HZ = 2.4e9
 
 
 class Hip:
@@ -102,6 +105,29 @@ def hitter_pmc_flops():
     with open(path) as f:
         t = json.load(f)
     return float(t["valu_flops_fp32_per_dispatch"]) * float(t["flop_per_count"]), t.get("source")
+
+
+def valu_counts():
+    """SQ_INSTS_VALU (wave64 vector instructions) per dispatch of the 1 MP frame's hitter launches
+    (bounces 1-4 and the realigning hit) and fused bounce passes (bounces 1-4), from the committed
+    rocprofv3 --pmc summary; None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "valu_c2_counts.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def valu_roofline(kernel, insts, ms, n_cu):
+    """Achieved vector-instruction issue rate of `kernel` (instructions per launch over event-timed
+    milliseconds per launch) against the chip's issue peak."""
+    peak = n_cu * PEAK_CLOCK_HZ / 1e9  # G wave-instructions/s
+    rates = [i / (t * 1e-3) / 1e9 for i, t in zip(insts, ms)]
+    tot = sum(insts) / (sum(ms) * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "valu-issue", "unit": "G wave64 VALU instructions/s", "peak": round(peak, 1),
+            "achieved": round(tot, 1), "frac": round(tot / peak, 4),
+            "per_launch": [{"insts": int(i), "ms": round(t, 4), "achieved": round(r, 1), "frac": round(r / peak, 4)}
+                           for i, t, r in zip(insts, ms, rates)]}
 
 
 def live_counts(cam, hitter, tris, params) -> list[int]:
@@ -776,6 +802,17 @@ def main() -> int:
     achieved = flop_per_launch / avg_launch_s / 1e12
     traffic, traffic_src = hitter_traffic()
     pmc_flop, pmc_src = hitter_pmc_flops()
+    # vector-instruction issue of the hitter and the fused bounce pass: counted instructions per
+    # launch (committed PMC summary) over the event-timed launches of this run
+    vc = valu_counts()
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    valu_roofs = None
+    if vc and len(vc["k_hit_bv"]) == n_launch and len(vc["k_bounce"]) == D:
+        hit_ms = [float(np.mean(launch_ms[j::n_launch])) for j in range(n_launch)]  # launch j of every frame
+        pass_ms_j = [float(np.mean([x[j] for x in pass_ms])) for j in range(D)]
+        valu_roofs = {"hitter": valu_roofline("k_hit_bv", vc["k_hit_bv"], hit_ms, n_cu),
+                      "bounce_pass": valu_roofline("k_bounce", vc["k_bounce"], pass_ms_j, n_cu),
+                      "source": vc.get("source")}
 
     splat_op = splat_operator(sc, cam, hitter, frame.tris, params, lm)
     part_op = partition_operator(cam, hitter, frame.tris, params)
@@ -850,6 +887,7 @@ def main() -> int:
                             "diffuse_rays": n_diffuse},
             "splat_operator": splat_op,
             "partition_operator": part_op,
+            "valu_issue": valu_roofs,
         },
         "hitter": {
             "ms_per_frame": round(hitter_frame_ms, 4),

@@ -246,8 +246,9 @@ __device__ __forceinline__ void b_hit_item(const EngineDev& E, const int32_t* __
 // QUEUES contiguous ranges, each with a device counter of the items taken (zeroed by the planning
 // kernel, on a cache line of its own); a wave takes chunks from the range of its workgroup's
 // XCD (workgroups go to the XCDs in turn), then from the others', each chunk an eighth of its
-// range's remaining share of a wave (single items at the end); it leaves when every range is
-// taken. (One counter for all items: every take serialises on one address, about 10 ns each,
+// range's remaining share of a wave (single items at the end), and asks for its next chunk
+// before it works on the current one (the atomic's round trip overlaps the work); it leaves when
+// every range is taken. (One counter for all items: every take serialises on one address, about 10 ns each,
 // the 64-frame batch 2.2x slower with single items; larger chunks leave long items queued behind
 // each other at the late bounces.)
 __global__ __launch_bounds__(256, 8) void k_b_hit(EngineDev E, int32_t use_queue) {
@@ -255,32 +256,37 @@ __global__ __launch_bounds__(256, 8) void k_b_hit(EngineDev E, int32_t use_queue
   const int32_t* P = E.pre + K_WAVE * (MAX_SLOTS + 1);
   const int32_t total = P[n];
   HitStats hs;
-  const int32_t W = (int32_t)gridDim.x * 4;
-  const int32_t w = (int32_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  int32_t i = w, end = use_queue ? w : (w + 1 < total ? w + 1 : total);
+  if (!use_queue) {
+    const int32_t w = (int32_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w < total) b_hit_item(E, P, n, w, hs);
+    return;
+  }
+  const int32_t Wq = (int32_t)gridDim.x * 4 / QUEUES > 0 ? (int32_t)gridDim.x * 4 / QUEUES : 1;
   int32_t q = (int32_t)(blockIdx.x % QUEUES), tried = 0, taken = 0;  // taken: range q's count as last seen
-  const int32_t Wq = W / QUEUES > 0 ? W / QUEUES : 1;
+  auto lo_of = [&](int32_t r) { return (int32_t)((int64_t)total * r / QUEUES); };
+  // a take from range q: its chunk size and the count before it (lane 0's atomic; read later, so
+  // the take is in flight while the wave works on the chunk before)
+  int32_t chunk = 0, at_raw = 0;
+  auto take = [&]() {
+    const int32_t rest = lo_of(q + 1) - lo_of(q) - taken;
+    chunk = rest > 8 * Wq ? rest / (8 * Wq) : 1;
+    if ((threadIdx.x & 63) == 0) at_raw = atomicAdd(E.queue + q * QUEUE_STRIDE, chunk);
+  };
+  take();
   while (true) {
-    if (i >= end) {
-      if (!use_queue) break;
-      const int32_t lo = (int32_t)((int64_t)total * q / QUEUES), hi = (int32_t)((int64_t)total * (q + 1) / QUEUES);
-      const int32_t rest = hi - lo - taken;
-      const int32_t chunk = rest > 8 * Wq ? rest / (8 * Wq) : 1;
-      int32_t at = 0;
-      if ((threadIdx.x & 63) == 0) at = atomicAdd(E.queue + q * QUEUE_STRIDE, chunk);
-      at = __builtin_amdgcn_readfirstlane(at);
-      if (at >= hi - lo) {  // range q is taken: the next one
-        if (++tried == QUEUES) break;
-        q = q + 1 < QUEUES ? q + 1 : 0;
-        taken = 0;
-        continue;
-      }
-      i = lo + at;
-      end = i + chunk < hi ? i + chunk : hi;
-      taken = at + chunk;
+    const int32_t at = __builtin_amdgcn_readfirstlane(at_raw), c = chunk;
+    const int32_t lo = lo_of(q), hi = lo_of(q + 1);
+    if (at >= hi - lo) {  // range q is taken: the next one
+      if (++tried == QUEUES) break;
+      q = q + 1 < QUEUES ? q + 1 : 0;
+      taken = 0;
+      take();
+      continue;
     }
-    b_hit_item(E, P, n, i, hs);
-    ++i;
+    taken = at + c;
+    take();  // the next chunk, requested before this one's work
+    const int32_t end = lo + at + c < hi ? lo + at + c : hi;
+    for (int32_t i = lo + at; i < end; ++i) b_hit_item(E, P, n, i, hs);
   }
 }
 
