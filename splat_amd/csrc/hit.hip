@@ -152,7 +152,7 @@ __global__ __launch_bounds__(64 * W) void k_hit_bf(const splat_adray_t* __restri
 // live count on the device, so bounces with few, incoherent survivors still spread over the whole
 // chip; the splits of a group merge their (t, index) keys with a 64-bit atomic minimum. The grid
 // covers the most items any live count can need (max_hit_items_one); waves past the count leave.
-template <bool STATS, int MODE>
+template <bool STATS, int MODE, bool STRIDE>
 __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict__ rays, int64_t R,
                                                 const int64_t* __restrict__ d_count, BVScene scene,
                                                 uint64_t* __restrict__ keys, unsigned long long* stats,
@@ -162,17 +162,29 @@ __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict_
   const int32_t units = MODE == BV_TREE ? (1 << scene.levels) : MODE == BV_GROUPS ? scene.n_groups : scene.n_bv;
   const bool tail = MODE == BV_TREE && groups64 <= TAIL_GROUPS && !scene.no_tail;  // few rays: flat over the leaves
   const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units);
-  const int64_t i = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (i >= groups64 * S) return;  // whole waves
+  const int64_t items = groups64 * S;
   HitStats st;
-  const int64_t g = i / S;
-  const int32_t sp = (int32_t)(i - g * S);
-  if (tail) {
-    if (from_cam) bv_item<STATS, BV_FLAT, true>(rays, A, g * 64, sp, S, scene, keys, st, &cam);
-    else bv_item<STATS, BV_FLAT, false>(rays, A, g * 64, sp, S, scene, keys, st);
+  const int64_t first = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if constexpr (STRIDE) {
+    // a capped grid (HIT_GRID_CAP) after the first bounce of a large frame: the items stride over it
+    // (no camera rays here: the first bounce's grid holds all its items)
+    for (int64_t i = first; i < items; i += (int64_t)gridDim.x * 4) {  // whole waves
+      const int64_t g = i / S;
+      const int32_t sp = (int32_t)(i - g * S);
+      if (tail) bv_item<STATS, BV_FLAT, false>(rays, A, g * 64, sp, S, scene, keys, st);
+      else bv_item<STATS, MODE, false>(rays, A, g * 64, sp, S, scene, keys, st);
+    }
   } else {
-    if (from_cam) bv_item<STATS, MODE, true>(rays, A, g * 64, sp, S, scene, keys, st, &cam);
-    else bv_item<STATS, MODE, false>(rays, A, g * 64, sp, S, scene, keys, st);
+    if (first >= items) return;  // whole waves
+    const int64_t g = first / S;
+    const int32_t sp = (int32_t)(first - g * S);
+    if (tail) {
+      if (from_cam) bv_item<STATS, BV_FLAT, true>(rays, A, g * 64, sp, S, scene, keys, st, &cam);
+      else bv_item<STATS, BV_FLAT, false>(rays, A, g * 64, sp, S, scene, keys, st);
+    } else {
+      if (from_cam) bv_item<STATS, MODE, true>(rays, A, g * 64, sp, S, scene, keys, st, &cam);
+      else bv_item<STATS, MODE, false>(rays, A, g * 64, sp, S, scene, keys, st);
+    }
   }
   if constexpr (STATS) {
     atomicAdd(stats + 0, st.n_box);
@@ -359,18 +371,27 @@ void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
                    uint64_t* stats, hipStream_t s, const splat_pinhole_t* cam) {
   if (R <= 0) return;
   auto* st = reinterpret_cast<unsigned long long*>(stats);
-  dim3 grid((unsigned)((max_hit_items_one((R + 63) / 64) + 3) / 4)), block(256);
+  // a workgroup (4 waves) per item any live count can need; past the first bounce (whose rays
+  // from the pinhole are all live) at most HIT_GRID_CAP, the items striding over it, so that a
+  // launch over few live rays of a large frame dispatches no more (a separate kernel: the loop
+  // costs the one-item kernel 2% of a 1 MP frame)
+  const int64_t wgs = (max_hit_items_one((R + 63) / 64) + 3) / 4;
+  const bool stride = !cam && !st && wgs > HIT_GRID_CAP;
+  dim3 grid((unsigned)(stride ? HIT_GRID_CAP : wgs)), block(256);
   splat_pinhole_t c;
   if (cam) c = *cam;
   else std::memset(&c, 0, sizeof(c));
   const int32_t fc = cam ? 1 : 0;
-#define SPL_HIT_BV(ST, M) \
-  hipLaunchKernelGGL((k_hit_bv<ST, M>), grid, block, 0, s, rays, R, d_count, scene, keys, st, c, fc)
+#define SPL_HIT_BV(ST, M, SD) \
+  hipLaunchKernelGGL((k_hit_bv<ST, M, SD>), grid, block, 0, s, rays, R, d_count, scene, keys, st, c, fc)
+#define SPL_HIT_BV_MODE(M) \
+  if (st) SPL_HIT_BV(true, M, false); else if (stride) SPL_HIT_BV(false, M, true); else SPL_HIT_BV(false, M, false)
   switch (bv_mode(scene)) {
-    case BV_TREE: if (st) SPL_HIT_BV(true, BV_TREE); else SPL_HIT_BV(false, BV_TREE); break;
-    case BV_GROUPS: if (st) SPL_HIT_BV(true, BV_GROUPS); else SPL_HIT_BV(false, BV_GROUPS); break;
-    default: if (st) SPL_HIT_BV(true, BV_FLAT); else SPL_HIT_BV(false, BV_FLAT); break;
+    case BV_TREE: SPL_HIT_BV_MODE(BV_TREE); break;
+    case BV_GROUPS: SPL_HIT_BV_MODE(BV_GROUPS); break;
+    default: SPL_HIT_BV_MODE(BV_FLAT); break;
   }
+#undef SPL_HIT_BV_MODE
 #undef SPL_HIT_BV
 }
 

@@ -86,6 +86,10 @@ inline BVScene bv_scene(const splat_hitset_t* hs) {
 inline __host__ __device__ int bv_mode(const BVScene& s) {
   return s.nodes ? BV_TREE : (s.groups && s.gsize > 0 && s.n_groups > 0 ? BV_GROUPS : BV_FLAT);
 }
+// Largest grid (workgroups) of the single-frame hitter after the first bounce: four rounds of the
+// chip's wave slots; the items of a larger frame stride over it. A grid sized for every ray of a
+// 33.5M-ray frame dispatched 131,072 workgroups at each late bounce for a few thousand live rays.
+constexpr int64_t HIT_GRID_CAP = 8192;
 // BV hitter into packed keys (initialised by launch_key_init on the same stream)
 void launch_key_init(uint64_t* keys, int64_t R, const int64_t* d_count, hipStream_t s);
 void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count, const BVScene& scene, uint64_t* keys,

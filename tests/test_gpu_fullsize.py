@@ -114,3 +114,16 @@ def test_large_scene_invariants_and_sample():
     cam = scene.camera_c3(seed=3, height=512, width=512)
     fr, p = _frame(sc, cam, 6, abi.lightmap_params(512, 512), bv_depth=12)
     _check_sample(sc, cam, fr, p, n=2000)
+
+
+def test_large_scene_strided_hitter_sampled_vs_oracle():
+    """640^2 x 8 wavelengths = 3.3M rays of the ~178k-triangle scene: past 2M rays the hitter's
+    launches after the first bounce run on a capped grid whose items stride (a grid sized for
+    every ray would dispatch its workgroups for the few thousand rays left at the last bounces).
+    Same invariants, and a sample of 2000 rays traced alone by the oracle."""
+    sc = scene.scene_c3()
+    cam = scene.camera_c3(seed=4, height=640, width=640)
+    fr, p = _frame(sc, cam, 6, abi.lightmap_params(640, 640), bv_depth=12, seed=91)
+    assert fr.R == 640 * 640 * 8 and (fr.R + 63) // 64 > 4 * 8192  # items past the capped grid
+    _check_sample(sc, cam, fr, p, n=2000, seed=5)
+    assert fr.tex.sum().item() > 0
