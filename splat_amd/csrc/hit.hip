@@ -375,7 +375,15 @@ void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
   // from the pinhole are all live) at most HIT_GRID_CAP, the items striding over it, so that a
   // launch over few live rays of a large frame dispatches no more (a separate kernel: the loop
   // costs the one-item kernel 2% of a 1 MP frame)
-  const int64_t wgs = (max_hit_items_one((R + 63) / 64) + 3) / 4;
+  int64_t wgs = (max_hit_items_one((R + 63) / 64) + 3) / 4;
+  if (cam) {  // every ray live: exactly the first bounce's items (the kernel's own split choice)
+    const int64_t groups64 = (R + 63) / 64;
+    const int mode = bv_mode(scene);
+    const int32_t units = mode == BV_TREE ? (1 << scene.levels) : mode == BV_GROUPS ? scene.n_groups : scene.n_bv;
+    const bool tail = mode == BV_TREE && groups64 <= TAIL_GROUPS && !scene.no_tail;
+    const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units);
+    wgs = (groups64 * S + 3) / 4;
+  }
   const bool stride = !cam && !st && wgs > HIT_GRID_CAP;
   dim3 grid((unsigned)(stride ? HIT_GRID_CAP : wgs)), block(256);
   splat_pinhole_t c;

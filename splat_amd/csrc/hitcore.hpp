@@ -518,7 +518,7 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
 // launches; 24576 splits the first bounce's coherent rays, 8192 too few for bounces 2-3.)
 constexpr int64_t SPLIT_TARGET_ITEMS = 12288;
 constexpr int32_t MAX_SPLIT = 32;
-__device__ __forceinline__ int32_t split_factor(int64_t groups64, int32_t units) {
+inline __host__ __device__ int32_t split_factor(int64_t groups64, int32_t units) {
   int32_t S = 1;
   while (S < MAX_SPLIT && S < units && groups64 * S < SPLIT_TARGET_ITEMS) S *= 2;
   return S;
@@ -532,7 +532,7 @@ constexpr int32_t TAIL_MAX_SPLIT = 128;
 constexpr int64_t TAIL_ITEMS = 8192;  // at most this many flat-scan items per launch
 // split of `groups64` tail groups (in all) over scenes of `leaves` leaves: S * 8 <= leaves,
 // groups64 * S <= TAIL_ITEMS
-__device__ __forceinline__ int32_t tail_split_factor(int64_t groups64, int32_t leaves) {
+inline __host__ __device__ int32_t tail_split_factor(int64_t groups64, int32_t leaves) {
   int32_t S = 1;
   while (S < TAIL_MAX_SPLIT && S * 16 <= leaves && groups64 * (S * 2) <= TAIL_ITEMS) S *= 2;
   return S;
