@@ -395,11 +395,10 @@ def cpu_baseline(sc, cam, params, lm, budget_s: float = 12.0):
 
 def batched_vs_plain(reps: int = 3, tune_cache=None):
     """64 mixed frames (256^2-1024^2 rays, icosphere subdiv 2-5 + plane, 4 bounces): all in one
-    engine batch (every frame's work items in shared launches, one graph replay per step; the
-    hitter one persistent launch per step fed by device work queues) versus the same frames traced
-    one frame per launch sequence. Both keep each frame's light map only (the served product): no
-    final rays or hit indices on either side. Also the batch with one hitter wave per item over a
-    host-sized grid instead of the queues."""
+    engine batch (every frame's work items in shared launches, one graph replay per step) versus
+    the same frames traced one frame per launch sequence. Both keep each frame's light map only
+    (the served product): no final rays or hit indices on either side. Also the batch with each
+    step's hitter one persistent launch fed by device work queues (the engine's use_queue mode)."""
     import torch
 
     from splat_amd import api, serve
@@ -433,15 +432,15 @@ def batched_vs_plain(reps: int = 3, tune_cache=None):
         eng.close()
         return min(out), st.steps
 
-    tb, steps = batch_s(True)  # the product: each step's hitter one persistent launch over the work queues
-    td, _ = batch_s(False)  # the same steps with one hitter wave per item over a host-sized grid
+    tb, steps = batch_s(False)  # the default: one hitter wave per item over a host-sized grid
+    tq, _ = batch_s(True)  # each step's hitter one persistent launch over the device work queues
     tp = min(plain)
     return {"frames": len(reqs), "rays": rays, "plain_mrays_per_s": round(rays / tp / 1e6, 3),
             "autotuned": [{k: s.tuned[k] for k in ("depth", "walk")} if s.tuned else None for s in scenes],
             "batched_mrays_per_s": round(rays / tb / 1e6, 3), "batched_vs_plain": round(tp / tb, 3),
             "plain_s": round(tp, 4), "batched_s": round(tb, 4), "steps_per_batch": steps,
-            "hitter": "one persistent launch per step, items from per-XCD device work queues",
-            "batched_dispatch_mrays_per_s": round(rays / td / 1e6, 3)}
+            "batched_queue_mrays_per_s": round(rays / tq / 1e6, 3),
+            "batched_queue_vs_plain": round(tp / tq, 3)}
 
 
 def _sweep_point(st, reqs, rate, extra=None):

@@ -461,11 +461,12 @@ int32_t splat_engine_sync_steps(splat_engine_t* e, int32_t keep);
 int64_t splat_engine_arena_free(const splat_engine_t* e);
 /* on != 0 (default): each step replays one captured HIP graph; 0: plain stream launches. */
 int32_t splat_engine_use_graph(splat_engine_t* e, int32_t on);
-/* on != 0 (default): a step's hitter is one persistent launch (a round of workgroups on every CU)
- * whose waves take their (64-ray group, split) items of every frame from a device work queue;
- * 0: one wave per item over a grid the host sizes from the frames' ray counts. The results are
- * the same. No reference counterpart (the reference traces one frame per launch sequence; the
- * nearest is the dynamic-parallelism launcher, src/hitters.jl:592-612). */
+/* on != 0: a step's hitter is one persistent launch (a round of workgroups on every CU) whose
+ * waves take their (64-ray group, split) items of every frame from device work queues; 0
+ * (default, faster on the 64-frame batch and in serving): one wave per item over a grid the host
+ * sizes from the frames' ray counts. The results are the same. No reference counterpart (the
+ * reference traces one frame per launch sequence; the nearest is the dynamic-parallelism
+ * launcher, src/hitters.jl:592-612). */
 int32_t splat_engine_use_queue(splat_engine_t* e, int32_t on);
 
 #ifdef __cplusplus

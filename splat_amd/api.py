@@ -448,7 +448,7 @@ class Engine:
     frames of different scenes, sizes and depths share every launch; a step advances each
     admitted frame one stage and replays one captured HIP graph."""
 
-    def __init__(self, arena_bytes: int, max_slots: int = 256, use_graph: bool = True, use_queue: bool = True):
+    def __init__(self, arena_bytes: int, max_slots: int = 256, use_graph: bool = True, use_queue: bool = False):
         L.require_gpu()
         self._lib = L.lib()
         h = C.c_void_p()

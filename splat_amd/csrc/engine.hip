@@ -239,16 +239,18 @@ __device__ __forceinline__ void b_hit_item(const EngineDev& E, const int32_t* __
 }
 
 // Hitter: one wave per (64-ray group, split) item over every frame's live rays; the splits of a
-// group merge into its keys with a 64-bit atomic minimum. Without the queue the grid (chosen by
-// the host from the frames' ray counts) covers the most items the live counts can need and waves
-// past them leave. With the queue (a persistent launch) the grid is one round of workgroups on
-// every CU and the waves take their items from the step's work queue: the items split into
-// QUEUES contiguous ranges, each with a device counter of the items taken (zeroed by the planning
+// group merge into its keys with a 64-bit atomic minimum. Without the queue (the default) the grid
+// (chosen by the host from the frames' ray counts) covers the most items the live counts can need
+// and waves past them leave. With the queue (a persistent launch) the grid is one round of workgroups on
+// every CU: each wave first takes the item of its own index, and the items past the grid (a
+// step with more items than waves) come from the step's work queue: they split into QUEUES
+// contiguous ranges, each with a device counter of the items taken (zeroed by the planning
 // kernel, on a cache line of its own); a wave takes chunks from the range of its workgroup's
 // XCD (workgroups go to the XCDs in turn), then from the others', each chunk an eighth of its
 // range's remaining share of a wave (single items at the end), and asks for its next chunk
 // before it works on the current one (the atomic's round trip overlaps the work); it leaves when
-// every range is taken. (One counter for all items: every take serialises on one address, about 10 ns each,
+// every range is taken. A step with no more items than waves never touches the queue (every wave
+// probing all the ranges cost serving steps 16% of their throughput). (One counter for all items: every take serialises on one address, about 10 ns each,
 // the 64-frame batch 2.2x slower with single items; larger chunks leave long items queued behind
 // each other at the late bounces.)
 __global__ __launch_bounds__(256, 8) void k_b_hit(EngineDev E, int32_t use_queue) {
@@ -256,14 +258,14 @@ __global__ __launch_bounds__(256, 8) void k_b_hit(EngineDev E, int32_t use_queue
   const int32_t* P = E.pre + K_WAVE * (MAX_SLOTS + 1);
   const int32_t total = P[n];
   HitStats hs;
-  if (!use_queue) {
-    const int32_t w = (int32_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (w < total) b_hit_item(E, P, n, w, hs);
-    return;
-  }
-  const int32_t Wq = (int32_t)gridDim.x * 4 / QUEUES > 0 ? (int32_t)gridDim.x * 4 / QUEUES : 1;
+  const int32_t w = (int32_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (w < total) b_hit_item(E, P, n, w, hs);  // a wave's first item is its own index
+  const int32_t W = (int32_t)gridDim.x * 4;
+  if (!use_queue || total <= W) return;  // no item past the grid: the queue is never touched
+  // the items past the grid, [W, total), from the work queue
+  const int32_t rem = total - W, Wq = W / QUEUES > 0 ? W / QUEUES : 1;
   int32_t q = (int32_t)(blockIdx.x % QUEUES), tried = 0, taken = 0;  // taken: range q's count as last seen
-  auto lo_of = [&](int32_t r) { return (int32_t)((int64_t)total * r / QUEUES); };
+  auto lo_of = [&](int32_t r) { return W + (int32_t)((int64_t)rem * r / QUEUES); };
   // a take from range q: its chunk size and the count before it (lane 0's atomic; read later, so
   // the take is in flight while the wave works on the chunk before)
   int32_t chunk = 0, at_raw = 0;
@@ -540,7 +542,7 @@ struct splat_engine {
   std::map<std::array<int64_t, 4>, hipGraphExec_t> graphs;  // one captured step per set of grid sizes
   hipStream_t cap_stream = nullptr;  // graphs are captured here (the null stream cannot capture)
   int use_graph = 1;
-  int use_queue = 1;                 // the hitter: a persistent grid taking items from a work queue
+  int use_queue = 0;                 // the hitter: a persistent grid taking items from a work queue
   int64_t queue_hit_blocks = 256 * 8;  // its grid: one round of workgroups on every CU
 };
 
