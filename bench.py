@@ -436,7 +436,7 @@ def batched_vs_plain(reps: int = 3, tune_cache=None):
     tq, _ = batch_s(True)  # each step's hitter one persistent launch over the device work queues
     tp = min(plain)
     return {"frames": len(reqs), "rays": rays, "plain_mrays_per_s": round(rays / tp / 1e6, 3),
-            "autotuned": [{k: s.tuned[k] for k in ("depth", "walk")} if s.tuned else None for s in scenes],
+            "autotuned": [{k: s.tuned.get(k) for k in ("depth", "walk", "split")} if s.tuned else None for s in scenes],
             "batched_mrays_per_s": round(rays / tb / 1e6, 3), "batched_vs_plain": round(tp / tb, 3),
             "plain_s": round(tp, 4), "batched_s": round(tb, 4), "steps_per_batch": steps,
             "batched_queue_mrays_per_s": round(rays / tq / 1e6, 3),
@@ -475,7 +475,7 @@ def served(n_requests: int = 120, rates=(40.0, 150.0, 300.0, 600.0, 1200.0), tun
     eng.close()
     out = dict(sweep[0])
     out.update({"hbm_budget_gib": 96, "triangles": [s.sc.n_tris for s in scenes], "sweep": sweep,
-                "autotuned": [{k: s.tuned[k] for k in ("depth", "walk")} if s.tuned else None for s in scenes]})
+                "autotuned": [{k: s.tuned.get(k) for k in ("depth", "walk", "split")} if s.tuned else None for s in scenes]})
     return out
 
 

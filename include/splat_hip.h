@@ -129,7 +129,10 @@ typedef struct splat_hitset {
                          /* BV tree: 0 = waves walk together, 1 = per wave, together when */
                          /* its rays share a direction octant, else per lane; 2 = per lane; */
                          /* + 4: walk the tree also when few rays are live (by default a   */
-                         /* launch with <= 4096 live rays scans the leaf boxes flat)         */
+                         /* launch with <= 4096 live rays scans the leaf boxes flat);        */
+                         /* BV, single-frame launches: + 256 x c, c = 1..4: the split factor */
+                         /* aims for 3072, 6144, 24576 or 49152 (64-ray group, split) items  */
+                         /* instead of 12288 (c = 0)                                         */
   int32_t tree_levels;   /* 1 + levels of the tree over the BVs; 0 = no tree (BV only)    */
   int32_t top_bvs;       /* leading BVs whose members every ray tests directly (BV only) */
   int32_t top_recs;      /* their records: d_recs[0, top_recs)                            */

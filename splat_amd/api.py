@@ -113,7 +113,8 @@ class HIPBVHitter:
         makes on the host, with the large-triangle list always present (possibly empty) and
         empty leaves kept. depth defaults to bv.tree_depth of all the triangles. `walk` is the
         tree walk mode (HitSet.tune_waves: 0 waves together, 1 per wave by direction octant, 2 per
-        lane; + 4 never the flat tail scan), kept across rebuilds."""
+        lane; + 4 never the flat tail scan; + 256 x the split-target code of autotune.SPLIT_TARGETS),
+        kept across rebuilds."""
         L.require_gpu()
         _check_rows(tris, abi.FTRI.itemsize, "tris")
         self = cls.__new__(cls)

@@ -11,7 +11,9 @@ Tuned knobs:
                  level of fixed-size groups);
   device BV   -- for the tree built on the device (resident and served scenes): leaf count 2^d
                  and the walk mode (waves walk together, per wave by direction octant, or per
-                 lane), timed on whole frames of a representative camera, all bounces included.
+                 lane), then, for the best of those, the work items a launch's split factor
+                 aims for (64-ray groups x splits: 3072 ... 49152), all timed on whole frames of
+                 a representative camera, all bounces included.
 All candidates return identical hit indices (the tuner asserts it on the tuning rays).
 """
 from __future__ import annotations
@@ -114,56 +116,74 @@ def tune_bv(tris_dev, host_tris: np.ndarray, rays, cache_path: str | None = None
     return choice
 
 
+# split-target codes of HitSet.tune_waves bits 8-10 (items the split factor aims for)
+SPLIT_TARGETS = {0: 12288, 1: 3072, 2: 6144, 3: 24576, 4: 49152}
+
+
 def tune_device_bv(tris_dev, cam, params, lm, cache_path: str | None = None, depths: tuple[int, ...] | None = None,
-                   walks: tuple[int, ...] = (0, 1, 2), reps: int = 3, frames_per_rep: int = 4) -> dict:
+                   walks: tuple[int, ...] = (0, 1, 2), targets: tuple[int, ...] = (0, 2, 3),
+                   reps: int = 3, frames_per_rep: int = 4) -> dict:
     """Best (leaf depth, walk mode) of the device-built tree hitter for frames like `cam` over this
-    scene: each candidate traces whole frames (pinhole rays, params.depth bounces, light map) and
-    the fastest median frame time wins. Returns {"depth", "walk", "ms", "candidates"} and the
-    winning hitter under "hitter" (not cached). Every candidate's final hit indices must agree."""
+    scene, then the best split target (code of SPLIT_TARGETS) for that pair: each candidate traces
+    whole frames (pinhole rays, params.depth bounces, light map) and the fastest median frame time
+    wins. Returns {"depth", "walk", "split", "ms", "candidates"} and the winning hitter under
+    "hitter" (not cached). Every candidate's final hit indices must agree."""
     import torch
 
     _load(cache_path)
     N = int(tris_dev.shape[0])
     R = cam.height * cam.width * cam.samples_per_pixel
-    key = _key(f"bvdev:d{params.depth}", R, N)
+    key = _key(f"bvdev2:d{params.depth}", R, N)
     if key not in _cache:
         if depths is None:
             d0 = bv.tree_depth(N - 1)
             depths = tuple(d for d in (d0 - 2, d0 - 1, d0, d0 + 1) if 1 <= d <= 20)
         frame = None
         best, ref, tried = None, None, []
+
+        def run(h, d, w, c):
+            nonlocal frame, best, ref
+            h.walk = w | (c << 8)
+            h.hs.tune_waves = h.walk
+            if frame is None:
+                frame = api.Frame(cam, h, tris_dev, params, lm, device=tris_dev.device)
+            frame.hitter = h
+            frame.trace()  # warm-up
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(reps):  # frames back to back, as a frame loop issues them
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(frames_per_rep):
+                    frame.trace()
+                b.record()
+                torch.cuda.synchronize()
+                ts.append(a.elapsed_time(b) / frames_per_rep)
+            hits = frame.hit_idx.cpu().numpy()
+            if ref is None:
+                ref = hits
+            elif not np.array_equal(ref, hits):
+                raise RuntimeError(f"device BV candidate depth={d} walk={w} split={c} changed the hit indices")
+            ms = float(np.median(ts))
+            tried.append([d, w, c, round(ms, 4)])
+            if best is None or ms < best["ms"]:
+                best = {"depth": d, "walk": w, "split": c, "ms": ms}
+
         for d in depths:
             h = api.HIPBVHitter.build_on_device(tris_dev, d)
             for w in walks:
-                h.walk = w
-                h.hs.tune_waves = w
-                if frame is None:
-                    frame = api.Frame(cam, h, tris_dev, params, lm, device=tris_dev.device)
-                frame.hitter = h
-                frame.trace()  # warm-up
-                torch.cuda.synchronize()
-                ts = []
-                for _ in range(reps):  # frames back to back, as a frame loop issues them
-                    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    a.record()
-                    for _ in range(frames_per_rep):
-                        frame.trace()
-                    b.record()
-                    torch.cuda.synchronize()
-                    ts.append(a.elapsed_time(b) / frames_per_rep)
-                hits = frame.hit_idx.cpu().numpy()
-                if ref is None:
-                    ref = hits
-                elif not np.array_equal(ref, hits):
-                    raise RuntimeError(f"device BV candidate depth={d} walk={w} changed the hit indices")
-                ms = float(np.median(ts))
-                tried.append([d, w, round(ms, 4)])
-                if best is None or ms < best["ms"]:
-                    best = {"depth": d, "walk": w, "ms": ms}
+                run(h, d, w, 0)
             del h
+        d, w = best["depth"], best["walk"]
+        h = api.HIPBVHitter.build_on_device(tris_dev, d)
+        for c in targets:
+            if c != 0:
+                run(h, d, w, c)
+        del h
         best["candidates"] = tried
         _cache[key] = best
         _save(cache_path)
     choice = dict(_cache[key])
-    choice["hitter"] = api.HIPBVHitter.build_on_device(tris_dev, choice["depth"], walk=choice["walk"])
+    choice["hitter"] = api.HIPBVHitter.build_on_device(tris_dev, choice["depth"],
+                                                      walk=choice["walk"] | (choice.get("split", 0) << 8))
     return choice

@@ -160,9 +160,9 @@ int32_t run_core(splat_adray_t* final_rays, int64_t R, const splat_hitset_t* hs,
   uint64_t* kc = w.keys;   // this bounce's keys
   uint64_t* kn = w.keys2;  // the next bounce's, set to the miss key by the bounce pass
   // the first bounce's keys need the miss key first unless every one is stored by the hitter
-  // itself: a BV launch over at least SPLIT_TARGET_ITEMS 64-ray groups runs one item per group
+  // itself: a BV launch over at least its split target's 64-ray groups runs one item per group
   // (split factor 1), which stores its rays' keys instead of merging them
-  const bool keys_stored = (R + 63) / 64 >= SPLIT_TARGET_ITEMS;
+  const bool keys_stored = bv && (R + 63) / 64 >= bv_scene(hs).split_target;
   launch_frame_start(lm ? tex : nullptr, lm ? (int64_t)lm->width * lm->height * lm->channels : 0, &w.counters[0], R,
                      bv && !keys_stored ? kc : nullptr, s);
   for (int k = 1; k <= p->depth; ++k) {

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_hit_bv(const splat_adray_t* __restrict_
   const int64_t groups64 = (A + 63) / 64;
   const int32_t units = MODE == BV_TREE ? (1 << scene.levels) : MODE == BV_GROUPS ? scene.n_groups : scene.n_bv;
   const bool tail = MODE == BV_TREE && groups64 <= TAIL_GROUPS && !scene.no_tail;  // few rays: flat over the leaves
-  const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units);
+  const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units, scene.split_target);
   const int64_t items = groups64 * S;
   HitStats st;
   const int64_t first = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -375,13 +375,13 @@ void launch_hit_bv(const splat_adray_t* rays, int64_t R, const int64_t* d_count,
   // from the pinhole are all live) at most HIT_GRID_CAP, the items striding over it, so that a
   // launch over few live rays of a large frame dispatches no more (a separate kernel: the loop
   // costs the one-item kernel 2% of a 1 MP frame)
-  int64_t wgs = (max_hit_items_one((R + 63) / 64) + 3) / 4;
+  int64_t wgs = (max_hit_items_one((R + 63) / 64, scene.split_target) + 3) / 4;
   if (cam) {  // every ray live: exactly the first bounce's items (the kernel's own split choice)
     const int64_t groups64 = (R + 63) / 64;
     const int mode = bv_mode(scene);
     const int32_t units = mode == BV_TREE ? (1 << scene.levels) : mode == BV_GROUPS ? scene.n_groups : scene.n_bv;
     const bool tail = mode == BV_TREE && groups64 <= TAIL_GROUPS && !scene.no_tail;
-    const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units);
+    const int32_t S = tail ? tail_split_factor(groups64, scene.n_bv) : split_factor(groups64, units, scene.split_target);
     wgs = (groups64 * S + 3) / 4;
   }
   const bool stride = !cam && !st && wgs > HIT_GRID_CAP;

@@ -518,9 +518,10 @@ __device__ __forceinline__ int32_t bv_traverse(const RayLanes& L, bool valid, co
 // launches; 24576 splits the first bounce's coherent rays, 8192 too few for bounces 2-3.)
 constexpr int64_t SPLIT_TARGET_ITEMS = 12288;
 constexpr int32_t MAX_SPLIT = 32;
-inline __host__ __device__ int32_t split_factor(int64_t groups64, int32_t units) {
+inline __host__ __device__ int32_t split_factor(int64_t groups64, int32_t units,
+                                                int64_t target = SPLIT_TARGET_ITEMS) {
   int32_t S = 1;
-  while (S < MAX_SPLIT && S < units && groups64 * S < SPLIT_TARGET_ITEMS) S *= 2;
+  while (S < MAX_SPLIT && S < units && groups64 * S < target) S *= 2;
   return S;
 }
 // A launch with at most TAIL_GROUPS live 64-ray groups (the last bounces: a few thousand rays
@@ -549,8 +550,8 @@ inline __host__ __device__ int64_t max_hit_items(int64_t groups64, int64_t tail_
 }
 // The same for a launch over one frame, whose items are all tree walks or all a flat-scan tail:
 // the larger bound, not the sum (every wave past the items still has to be dispatched).
-inline __host__ __device__ int64_t max_hit_items_one(int64_t groups64) {
-  const int64_t walk = groups64 > 2 * SPLIT_TARGET_ITEMS ? groups64 : 2 * SPLIT_TARGET_ITEMS;
+inline __host__ __device__ int64_t max_hit_items_one(int64_t groups64, int64_t target = SPLIT_TARGET_ITEMS) {
+  const int64_t walk = groups64 > 2 * target ? groups64 : 2 * target;
   return walk > TAIL_ITEMS ? walk : TAIL_ITEMS;
 }
 

@@ -56,11 +56,14 @@ struct BVScene {
   int32_t walk;     // tree only: BV_WALK_*
   int32_t n_top;    // records recs[0, n_top) that every ray tests before the structure
   int32_t no_tail;  // tree only: walk the tree even for launches with few live rays
+  int32_t split_target;  // single-frame launches: items the split factor aims for (SPLIT_TARGETS)
 };
 // How a wave walks the tree: together (one path, the union of its rays'; scalar node loads), or
 // each lane its own path; AUTO: together when all its rays point into one octant.
 enum BVWalk { BV_WALK_WAVE = 0, BV_WALK_AUTO = 1, BV_WALK_LANES = 2 };
 enum BVMode { BV_FLAT = 0, BV_GROUPS = 1, BV_TREE = 2 };
+// split targets of the single-frame BV hitter (splat_hitset_t.tune_waves bits 8-10; 0: the default)
+constexpr int32_t SPLIT_TARGETS[8] = {12288, 3072, 6144, 24576, 49152, 12288, 12288, 12288};
 // the kernels' view of a prepared BV hit set
 inline BVScene bv_scene(const splat_hitset_t* hs) {
   BVScene b;
@@ -80,6 +83,7 @@ inline BVScene bv_scene(const splat_hitset_t* hs) {
   const int32_t w = hs->tune_waves & 3;
   b.walk = tree && w <= BV_WALK_LANES ? w : BV_WALK_WAVE;
   b.no_tail = tree && (hs->tune_waves & 4) ? 1 : 0;
+  b.split_target = SPLIT_TARGETS[(hs->tune_waves >> 8) & 7];
   return b;
 }
 

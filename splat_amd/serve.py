@@ -41,7 +41,8 @@ class ResidentScene:
             return ResidentScene(sc, tris, api.HIPBVHitter.build_on_device(tris, depth), sc.name)
         cam = lights_cam(1, 512, 1, abi.LAMBDA_UNIFORM)
         p = abi.run_params(4, sc.first_diffuse, math.nan, seed=1)
-        ch = autotune.tune_device_bv(tris, cam, p, abi.lightmap_params(512, 512), cache_path=tune_cache)
+        # (the engine sets its split factors for all its frames at once: no split target to tune)
+        ch = autotune.tune_device_bv(tris, cam, p, abi.lightmap_params(512, 512), cache_path=tune_cache, targets=(0,))
         tuned = {k: v for k, v in ch.items() if k != "hitter"}
         return ResidentScene(sc, tris, ch["hitter"], sc.name, tuned)
 

@@ -55,7 +55,33 @@ def test_autotune_device_bv_frames(tmp_path):
     torch.cuda.synchronize()
     assert torch.equal(base.hit_idx, tuned.hit_idx) and torch.equal(base.rays, tuned.rays)
     again = autotune.tune_device_bv(tris, cam, p, lm, cache_path=path)
-    assert (again["depth"], again["walk"]) == (ch["depth"], ch["walk"])
+    assert (again["depth"], again["walk"], again["split"]) == (ch["depth"], ch["walk"], ch["split"])
+    assert ch["split"] in autotune.SPLIT_TARGETS and len(ch["candidates"]) >= 8  # the split targets too
     assert "bvdev" in "".join(json.load(open(path)))
     ch["hitter"].rebuild(tris)
-    assert ch["hitter"].hs.tune_waves == ch["walk"]
+    assert ch["hitter"].hs.tune_waves == ch["walk"] | (ch["split"] << 8)
+
+
+@pytest.mark.parametrize("split", [1, 2, 3, 4])
+def test_split_targets_equal_default(split):
+    """Every split target of the single-frame BV hitter (HitSet.tune_waves bits 8-10) traces the
+    same frame as the default: a 128x128 frame is 256 64-ray groups, so each target gives the
+    bounces different split factors (and 3072 / 6144 the first bounce's keys merged, not stored)."""
+    import math
+
+    import torch
+
+    from splat_amd import abi
+
+    sc = H.small_c2(2)
+    tris = api.to_device(sc.ftris)
+    cam = scene.camera_c2(seed=12, height=128, width=128)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=8)
+    lm = abi.lightmap_params(64, 64)
+    base = api.Frame(cam, api.HIPBVHitter.build_on_device(tris, 6), tris, p, lm)
+    base.trace()
+    other = api.Frame(cam, api.HIPBVHitter.build_on_device(tris, 6, walk=split << 8), tris, p, lm)
+    other.trace()
+    torch.cuda.synchronize()
+    assert torch.equal(base.hit_idx, other.hit_idx) and torch.equal(base.rays, other.rays)
+    assert H.rel_l2(other.tex.cpu().numpy(), base.tex.cpu().numpy()) < 1e-4
