@@ -83,51 +83,40 @@ def build_workload(device="cuda", tune=True, tune_cache=None):
     return sc, cam, tuned, hitter, params, lm, frame
 
 
-def hitter_traffic():
-    """HBM bytes per hitter launch on this workload, from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same counters read for a device
-    copy of known size); None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_hit_bv_c2_traffic.json")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as f:
-        t = json.load(f)
-    return float(t["hbm_bytes_per_launch"]), t.get("source")
-
-
-def hitter_pmc_flops():
-    """FP32 FLOP per hitter launch on this workload from the committed rocprofv3 --pmc summary:
-    SQ_INSTS_VALU_FLOPS_FP32 per dispatch times the factor the same counter reads for the
-    calibration kernel of known FLOP count (bench.py --flop-probe); None when absent."""
-    path = os.path.join(ROOT, "profiles", "k_hit_bv_c2_counters.json")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as f:
-        t = json.load(f)
-    return float(t["valu_flops_fp32_per_dispatch"]) * float(t["flop_per_count"]), t.get("source")
-
-
-def valu_counts():
-    """SQ_INSTS_VALU (wave64 vector instructions) per dispatch of the 1 MP frame's hitter launches
-    (bounces 1-4 and the realigning hit) and fused bounce passes (bounces 1-4), from the committed
-    rocprofv3 --pmc summary; None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "valu_c2_counts.json")
+def _profile_json(name):
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)
 
 
-def valu_roofline(kernel, insts, ms, n_cu):
-    """Achieved vector-instruction issue rate of `kernel` (instructions per launch over event-timed
-    milliseconds per launch) against the chip's issue peak."""
-    peak = n_cu * PEAK_CLOCK_HZ / 1e9  # G wave-instructions/s
-    rates = [i / (t * 1e-3) / 1e9 for i, t in zip(insts, ms)]
-    tot = sum(insts) / (sum(ms) * 1e-3) / 1e9
-    return {"kernel": kernel, "bound": "valu-issue", "unit": "G wave64 VALU instructions/s", "peak": round(peak, 1),
-            "achieved": round(tot, 1), "frac": round(tot / peak, 4),
-            "per_launch": [{"insts": int(i), "ms": round(t, 4), "achieved": round(r, 1), "frac": round(r / peak, 4)}
-                           for i, t, r in zip(insts, ms, rates)]}
+def hitter_traffic(depth):
+    """HBM bytes per hitter launch on this workload at this leaf depth, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what the same
+    counters read for a device copy of known size); None when absent for that depth."""
+    t = _profile_json("k_hit_bv_c2_traffic.json")
+    e = t and t["by_leaf_depth"].get(str(depth))
+    return (float(e["hbm_bytes_per_launch"]), t.get("source")) if e else (None, None)
+
+
+def hitter_pmc_flops(depth):
+    """FP32 FLOP per hitter launch on this workload at this leaf depth from the committed
+    rocprofv3 --pmc summary: SQ_INSTS_VALU_FLOPS_FP32 per dispatch times the factor the same
+    counter reads for the calibration kernel of known FLOP count (bench.py --flop-probe); None
+    when absent for that depth."""
+    t = _profile_json("k_hit_bv_c2_counters.json")
+    e = t and t["by_leaf_depth"].get(str(depth))
+    return (float(e["valu_flops_fp32_per_dispatch"]) * float(t["flop_per_count"]), t.get("source")) if e else (None, None)
+
+
+def valu_counts(depth):
+    """SQ_INSTS_VALU (wave64 vector instructions) per dispatch of the 1 MP frame's hitter launches
+    (bounces 1-4 and the realigning hit) and fused bounce passes (bounces 1-4) at this leaf depth,
+    from the committed rocprofv3 --pmc summary; None when absent for that depth."""
+    t = _profile_json("valu_c2_counts.json")
+    e = t and t["by_leaf_depth"].get(str(depth))
+    return dict(e, source=t.get("source")) if e else None
 
 
 def live_counts(cam, hitter, tris, params) -> list[int]:
@@ -799,11 +788,11 @@ def main() -> int:
     flop_per_launch = (FLOP_PER_BOX_TEST * box_tests + FLOP_PER_TRI_TEST * alg_tests) / n_launch
     flop_exec = (FLOP_PER_BOX_TEST * box_exec + FLOP_PER_TRI_TEST * exec_tests) / n_launch
     achieved = flop_per_launch / avg_launch_s / 1e12
-    traffic, traffic_src = hitter_traffic()
-    pmc_flop, pmc_src = hitter_pmc_flops()
+    traffic, traffic_src = hitter_traffic(hitter.depth)
+    pmc_flop, pmc_src = hitter_pmc_flops(hitter.depth)
     # vector-instruction issue of the hitter and the fused bounce pass: counted instructions per
     # launch (committed PMC summary) over the event-timed launches of this run
-    vc = valu_counts()
+    vc = valu_counts(hitter.depth)
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
     valu_roofs = None
     if vc and len(vc["k_hit_bv"]) == n_launch and len(vc["k_bounce"]) == D:
