@@ -459,6 +459,11 @@ class Engine:
         L.check(self._lib.splat_engine_use_queue(self.h, 1 if use_queue else 0), "engine_use_queue")
         self._keep = {}  # tag -> objects that must outlive the frame
 
+    def set_use_queue(self, on: bool) -> None:
+        """Selects the hitter of the following steps: one persistent launch over the device work
+        queues (True) or one wave per item (False); the results are the same."""
+        L.check(self._lib.splat_engine_use_queue(self.h, 1 if on else 0), "engine_use_queue")
+
     def frame_bytes(self, cam: abi.Pinhole, with_outputs: bool = False) -> int:
         return int(self._lib.splat_engine_frame_bytes(L.ref(cam), 1 if with_outputs else 0))
 

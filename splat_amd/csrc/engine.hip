@@ -239,21 +239,29 @@ __device__ __forceinline__ void b_hit_item(const EngineDev& E, const int32_t* __
 }
 
 // Hitter: one wave per (64-ray group, split) item over every frame's live rays; the splits of a
-// group merge into its keys with a 64-bit atomic minimum. Without the queue (the default) the grid
-// (chosen by the host from the frames' ray counts) covers the most items the live counts can need
-// and waves past them leave. With the queue (a persistent launch) the grid is one round of workgroups on
-// every CU: each wave first takes the item of its own index, and the items past the grid (a
-// step with more items than waves) come from the step's work queue: they split into QUEUES
-// contiguous ranges, each with a device counter of the items taken (zeroed by the planning
-// kernel, on a cache line of its own); a wave takes chunks from the range of its workgroup's
-// XCD (workgroups go to the XCDs in turn), then from the others', each chunk an eighth of its
-// range's remaining share of a wave (single items at the end), and asks for its next chunk
-// before it works on the current one (the atomic's round trip overlaps the work); it leaves when
-// every range is taken. A step with no more items than waves never touches the queue (every wave
-// probing all the ranges cost serving steps 16% of their throughput). (One counter for all items: every take serialises on one address, about 10 ns each,
-// the 64-frame batch 2.2x slower with single items; larger chunks leave long items queued behind
-// each other at the late bounces.)
-__global__ __launch_bounds__(256, 8) void k_b_hit(EngineDev E, int32_t use_queue) {
+// group merge into its keys with a 64-bit atomic minimum. The grid (chosen by the host from the
+// frames' ray counts) covers the most items the live counts can need; waves past them leave.
+__global__ __launch_bounds__(256) void k_b_hit(EngineDev E) {
+  const int32_t n = E.n_slots;
+  const int32_t* P = E.pre + K_WAVE * (MAX_SLOTS + 1);
+  HitStats hs;
+  const int32_t w = (int32_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (w < P[n]) b_hit_item(E, P, n, w, hs);
+}
+
+// The same items from one persistent launch (splat_engine_use_queue): one round of workgroups on
+// every CU. Each wave first takes the item of its own index; the items past the grid (a step with
+// more items than waves) come from the step's work queue: they split into QUEUES contiguous
+// ranges, each with a device counter of the items taken (zeroed by the planning kernel, on a
+// cache line of its own); a wave takes chunks from the range of its workgroup's XCD (workgroups
+// go to the XCDs in turn), then from the others', each chunk an eighth of its range's remaining
+// share of a wave (single items at the end), and asks for its next chunk before it works on the
+// current one (the atomic's round trip overlaps the work); it leaves when every range is taken.
+// A step with no more items than waves never touches the queue: every wave probing all the
+// ranges cost serving steps 16% of their throughput. With one counter for all items every take
+// serialises on one address (about 10 ns each: the 64-frame batch 2.2x slower with single items);
+// larger chunks leave long late-bounce items queued behind each other in one wave.
+__global__ __launch_bounds__(256, 8) void k_b_hit_queue(EngineDev E) {
   const int32_t n = E.n_slots;
   const int32_t* P = E.pre + K_WAVE * (MAX_SLOTS + 1);
   const int32_t total = P[n];
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(256, 8) void k_b_hit(EngineDev E, int32_t use_queue
   const int32_t w = (int32_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (w < total) b_hit_item(E, P, n, w, hs);  // a wave's first item is its own index
   const int32_t W = (int32_t)gridDim.x * 4;
-  if (!use_queue || total <= W) return;  // no item past the grid: the queue is never touched
+  if (total <= W) return;  // no item past the grid: the queue is never touched
   // the items past the grid, [W, total), from the work queue
   const int32_t rem = total - W, Wq = W / QUEUES > 0 ? W / QUEUES : 1;
   int32_t q = (int32_t)(blockIdx.x % QUEUES), tried = 0, taken = 0;  // taken: range q's count as last seen
@@ -575,7 +583,8 @@ int32_t launch_step_kernels(splat_engine* e, const StepGrids& g, hipStream_t s) 
   const int64_t hit_blocks = g.hit_blocks;
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(MAX_SLOTS), 0, s, E);
   hipLaunchKernelGGL(k_b_raygen, dim3((unsigned)g.first_blocks), dim3(256), 0, s, E);
-  hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E, e->use_queue);
+  if (e->use_queue) hipLaunchKernelGGL(k_b_hit_queue, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
+  else hipLaunchKernelGGL(k_b_hit, dim3((unsigned)hit_blocks), dim3(256), 0, s, E);
   hipLaunchKernelGGL(k_b_count, dim3((unsigned)((g.tile_blocks + 3) / 4)), dim3(TILE), 0, s, E);
   hipLaunchKernelGGL(k_b_scan, dim3(e->max_slots), dim3(1024), 0, s, E);
   hipLaunchKernelGGL(k_b_bounce, dim3((unsigned)g.tile_blocks), dim3(TILE), 0, s, E);
@@ -802,7 +811,10 @@ int32_t splat_engine_step(splat_engine_t* e, void* stream) {
   // groups / 4 + slots, so this never passes hit_blocks
   grids.tile_blocks = grid_class(tiles);
   if (e->use_graph) {
-    hipGraphExec_t& ge = e->graphs[grids.key()];
+    // one graph per set of grids and hitter kernel (the mode can change between steps)
+    std::array<int64_t, 4> key = grids.key();
+    if (e->use_queue) key[0] = -key[0];
+    hipGraphExec_t& ge = e->graphs[key];
     if (!ge) {
       hipGraph_t g;
       if (!e->cap_stream && hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) != hipSuccess) return SPLAT_ERR_HIP;

@@ -159,6 +159,34 @@ def test_engine_many_flat_tails_beside_a_split_walk(scenes, use_queue):
     eng.close()
 
 
+def test_engine_hitter_mode_switch_between_steps(scenes):
+    """The hitter mode switched between the steps of one engine (graphs are kept per mode): the
+    188 small frames and the split walk of the test above, the first step on the work queues
+    (more items than the persistent grid's waves), the next without, then back; every frame equals
+    its single-frame trace."""
+    rng = np.random.default_rng(13)
+    specs = [dict(si=1, h=224, w=200, depth=3, fr=math.nan, seed=902, mode="rgb", nd=1, lmw=96, lmh=80)]
+    for k in range(188):
+        specs.append(dict(si=int(rng.integers(0, 3)), h=64, w=64, depth=int(rng.integers(1, 4)), fr=math.nan,
+                          seed=int(rng.integers(1, 1 << 30)), mode="mono", nd=1, lmw=48, lmh=40))
+    frames = [_make(s, scenes) for s in specs]
+    want = [_plain(f) for f in frames]
+    eng = api.Engine(1 << 30, max_slots=256, use_queue=True)
+    for k, f in enumerate(frames):
+        assert _admit(eng, f, 900 + k)
+    done, steps = [], 0
+    while len(done) < len(frames):
+        eng.set_use_queue(steps % 2 == 0)
+        eng.step()
+        steps += 1
+        done += eng.poll(wait=True)
+        assert steps <= 10
+    assert steps >= 3 and sorted(done) == [900 + k for k in range(len(frames))]
+    for f, w in zip(frames, want):
+        _check(f, w)
+    eng.close()
+
+
 def test_engine_continuous_admission_reuses_slots(scenes):
     """Frames arrive one per step while others are mid-flight; 3 slots serve 10 frames."""
     specs = _frames(10, seed=2)
