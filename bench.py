@@ -119,6 +119,18 @@ def valu_counts(depth):
     return dict(e, source=t.get("source")) if e else None
 
 
+def valu_roofline(kernel, insts, ms, n_cu):
+    """Achieved vector-instruction issue rate of `kernel` (instructions per launch over event-timed
+    milliseconds per launch) against the chip's issue peak."""
+    peak = n_cu * PEAK_CLOCK_HZ / 1e9  # G wave-instructions/s
+    rates = [i / (t * 1e-3) / 1e9 for i, t in zip(insts, ms)]
+    tot = sum(insts) / (sum(ms) * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "valu-issue", "unit": "G wave64 VALU instructions/s", "peak": round(peak, 1),
+            "achieved": round(tot, 1), "frac": round(tot / peak, 4),
+            "per_launch": [{"insts": int(i), "ms": round(t, 4), "achieved": round(r, 1), "frac": round(r / peak, 4)}
+                           for i, t, r in zip(insts, ms, rates)]}
+
+
 def live_counts(cam, hitter, tris, params) -> list[int]:
     """Live (ACTIVE) rays before each bounce of the frame, A_0 = R, ..., A_depth, from the same
     hitter / evolve / partition operators stepped one bounce at a time (identical rays and
