@@ -78,6 +78,8 @@ def test_argument_errors_without_a_gpu():
     lm.n_delta = 99
     assert lib.splat_light_map(C.c_void_p(8), 1, None, C.c_void_p(8), C.c_void_p(8), 3, L.ref(lm), C.c_void_p(8),
                                None) == -4
+    assert lib.splat_engine_use_queue(None, 1) == abi.SPLAT_ERR_INVALID_ARG
+    assert lib.splat_engine_use_graph(None, 1) == abi.SPLAT_ERR_INVALID_ARG
 
 
 def test_light_ray_counts_without_a_gpu():
