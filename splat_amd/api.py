@@ -418,6 +418,49 @@ class Frame:
                                           L.stream_handle(stream)), "trace_frame")
 
 
+class FrameLoop:
+    """The double-buffered frame loop of a scene script (src/scenes/sphere.jl:192-219, where frames
+    A and B are @async tasks, each on its task's own stream, so that one frame's launches run
+    while the other's finish), for `in_flight` buffers: each buffer is a Frame with its own stream.
+    `trace(cam)` issues the next frame on the next buffer's stream, after that buffer's previous
+    frame (the stream orders them), and returns the buffer without waiting; `wait(frame)` blocks
+    until that buffer's frames are done (`wait()`: every buffer's). A buffer's tex, rays and
+    hit_idx are those of its last frame: read them after `wait` and before the loop comes round to
+    that buffer again. A frame's launches fill the idle tails of the others' (a hitter launch ends
+    with a few long walks on an otherwise idle chip)."""
+
+    def __init__(self, cam: abi.Pinhole, hitter, tris, params: abi.RunParams, lm: abi.LightMapParams,
+                 in_flight: int = 2, device: str = "cuda", keep_rays: bool = True):
+        torch = _torch()
+        if in_flight < 1:
+            raise ValueError(f"in_flight must be at least 1, not {in_flight}")
+        self.frames = [Frame(cam, hitter, tris, params, lm, device, keep_rays) for _ in range(in_flight)]
+        self.streams = [torch.cuda.Stream(device=device) for _ in range(in_flight)]  # default priority
+        here = torch.cuda.current_stream(device)
+        for f, s in zip(self.frames, self.streams):
+            s.wait_stream(here)  # the buffers were allocated (the maps zeroed) on this stream
+            for t in (f.rays, f.hit_idx, f.tex, f.ws):
+                if t is not None:
+                    t.record_stream(s)  # freed only after the work queued on s when they go
+        self.next = 0
+
+    def trace(self, cam: abi.Pinhole | None = None) -> Frame:
+        i = self.next
+        self.next = (i + 1) % len(self.frames)
+        f = self.frames[i]
+        if cam is not None:
+            if cam.height * cam.width * cam.samples_per_pixel != f.R:
+                raise ValueError("a FrameLoop's cameras keep the ray count its buffers were sized for")
+            f.cam = cam  # passed by value at launch: the caller may change it after trace returns
+        f.trace(self.streams[i])
+        return f
+
+    def wait(self, frame: Frame | None = None) -> None:
+        for f, s in zip(self.frames, self.streams):
+            if frame is None or f is frame:
+                s.synchronize()
+
+
 class LightFrame:
     """The forward pass of trace! from rectangular lights (src/tracer.jl:316-341): rays_from_lights
     -> run_evolution! -> continuum_light_map2!, one call, no host sync."""

@@ -127,3 +127,35 @@ def test_large_scene_strided_hitter_sampled_vs_oracle():
     assert fr.R == 640 * 640 * 8 and (fr.R + 63) // 64 > 4 * 8192  # items past the capped grid
     _check_sample(sc, cam, fr, p, n=2000, seed=5)
     assert fr.tex.sum().item() > 0
+
+
+def test_frames_in_flight_on_their_own_streams_equal_their_solo_frames():
+    """FrameLoop (the double-buffered frame loop of src/scenes/sphere.jl:192-219) with three
+    buffers on three streams: six 512x512 frames of the 10,240-triangle scene from six cameras,
+    issued three at a time with no wait between them, so their launches overlap on the GPU. Each
+    buffer, read after its wait, holds exactly the rays and hit indices of the same camera's frame
+    traced alone, and the same light map within the float atomics' summation order; one frame's
+    rays are also sampled against the oracle."""
+    import torch
+
+    sc = scene.scene_c2(seed=2)
+    tris = api.to_device(sc.ftris)
+    hitter = api.HIPBVHitter.build_on_device(tris, depth=10)
+    p = abi.run_params(4, sc.first_diffuse, math.nan, seed=2024)
+    lm = abi.lightmap_params(512, 512)
+    cams = [scene.camera_c2(seed=s, height=512, width=512, jitter=0.025) for s in (3, 4, 5, 6, 7, 8)]
+    loop = api.FrameLoop(cams[0], hitter, tris, p, lm, in_flight=3)
+    solo = api.Frame(cams[0], hitter, tris, p, lm)
+    for first in (0, 3):
+        issued = [(cam, loop.trace(cam)) for cam in cams[first:first + 3]]
+        loop.wait()
+        assert len({id(f) for _, f in issued}) == 3
+        for cam, f in issued:
+            solo.cam = cam
+            solo.trace()
+            torch.cuda.synchronize()
+            assert torch.equal(f.hit_idx, solo.hit_idx) and torch.equal(f.rays, solo.rays)
+            assert solo.tex.sum().item() > 0 and H.rel_l2(f.tex.cpu().numpy(), solo.tex.cpu().numpy()) < 1e-5
+    _check_sample(sc, cams[5], issued[-1][1], p, n=1500, seed=4)
+    with pytest.raises(ValueError):
+        loop.trace(scene.camera_c2(seed=9, height=256, width=256))
