@@ -85,3 +85,12 @@ def test_a_request_larger_than_the_arena_is_an_error_not_a_wait():
     eng = StandInEngine(arena_bytes=48 * 48 * 100)
     with pytest.raises(ValueError, match="request 1"):
         serve.run_requests(eng, scenes, reqs, realtime=False)
+
+
+def test_frame_loop_needs_a_buffer():
+    """api.FrameLoop refuses fewer than one buffer before it allocates anything."""
+    from splat_amd import api
+
+    for n in (0, -2):
+        with pytest.raises(ValueError, match="in_flight"):
+            api.FrameLoop(scene.camera_c1(), None, None, None, None, in_flight=n)
