@@ -286,8 +286,10 @@ def bf_sweep(reps: int = 5):
 def splat_operator(sc, cam, hitter, tris, params, lm, reps: int = 20):
     """continuum_light_map2! as the standalone operator (splat_light_map: final evolution of the
     DIFFUSE rays, then the atomic splat) on the 1 MP frame's rays after run_evolution!; algorithmic
-    bytes: per diffuse ray its record's 109 used bytes and its 4-byte hit index, each touched
-    triangle's 120 bytes once, the map read and written once."""
+    bytes: per diffuse ray its record's 109 used bytes and its 4-byte hit index read, and the 52
+    bytes final_evolution changes written back (position and its three tangents, the hit index:
+    the reference assigns the evolved rays back, src/ray_imagers.jl:155), each touched triangle's
+    120 bytes once, the map read and written once."""
     import numpy as np
     import torch
 
@@ -316,7 +318,7 @@ def splat_operator(sc, cam, hitter, tris, params, lm, reps: int = 20):
         if k >= 2:
             ms.append(a.elapsed_time(b))
     m = float(np.median(ms))
-    nbytes = n_dif * (109 + 4) + touched * 120 + 2 * lm.width * lm.height * lm.channels * 4
+    nbytes = n_dif * (109 + 4 + 52) + touched * 120 + 2 * lm.width * lm.height * lm.channels * 4
     return {"kernel": "k_light_map (continuum_light_map2!: final evolution + atomic splat, standalone)",
             "bound": "hbm", "achieved": round(nbytes / (m / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(nbytes / (m / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nbytes, "median_ms": round(m, 4),
@@ -643,6 +645,9 @@ def main() -> int:
                     help="only the 320-triangle brute-force frame, --steps times (for a kernel profile)")
     ap.add_argument("--profile-batched", action="store_true",
                     help="only the 64-frame batch through the engine, --steps times (for a kernel profile)")
+    ap.add_argument("--profile-operators", action="store_true",
+                    help="only the standalone partition! and continuum_light_map2! on the 1 MP frame's rays, "
+                         "--steps timed calls each (for a kernel profile)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -714,6 +719,11 @@ def main() -> int:
         return 0
     sc, cam, tuned, hitter, params, lm, frame = build_workload(tune_cache=args.tune_cache)
     R = frame.R
+    if args.profile_operators:
+        reps = max(args.steps, 1)
+        print(json.dumps({"partition_operator": partition_operator(cam, hitter, frame.tris, params, reps=reps),
+                          "splat_operator": splat_operator(sc, cam, hitter, frame.tris, params, lm, reps=reps)}))
+        return 0
     for _ in range(args.warmup):
         frame.trace()
     torch.cuda.synchronize()
