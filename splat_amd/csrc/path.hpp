@@ -310,12 +310,15 @@ __device__ __forceinline__ void final_evolve_splat(RayF& r, int32_t i, const FTr
   }
 }
 
-// final evolution + splat of one DIFFUSE ray, written back to ray_out
+// final evolution + splat of one DIFFUSE ray loaded from ray_out, written back in place: final
+// evolution changes only the position, its three tangents and ignore_tri, so only those 56 of the
+// record's 136 bytes are stored (the rest of the record already holds the ray)
 __device__ __forceinline__ void final_evolve_and_splat(RayF r, int32_t i, const FTriF& F,
                                                        const splat_lightmap_params_t& lm, float* __restrict__ tex,
                                                        splat_adray_t* ray_out) {
   final_evolve_splat(r, i, F, lm, tex);
-  store_rayf(ray_out, r);
+  st3(ray_out->pos, r.pos); st3(ray_out->dpdl, r.dp[0]); st3(ray_out->dpdx, r.dp[1]); st3(ray_out->dpdy, r.dp[2]);
+  ray_out->ignore_tri = r.ignore_tri;
 }
 
 // ---- backward shading: shade_tex4_kernel (src/ray_imagers.jl:381-436) ------------------------
