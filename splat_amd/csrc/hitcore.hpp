@@ -85,11 +85,13 @@ __device__ __forceinline__ void take_nearer(bool h, float t, int32_t idx, float&
 // ds_read_b128 per record, no bank conflict). Scalar loads return out of order, so a loop of them
 // waits for each record in turn; this waits once per 16. (Broadcasting with v_readlane instead,
 // 16 per record, spent a quarter of a leaf scan's VALU issue: the LDS read is 4.5% faster per
-// frame.) Called by whole waves of a workgroup of at most 1024 threads; LDS operations of one wave
-// complete in order, so the reads see the wave's own writes without a barrier.
+// frame.) Called by whole waves of a workgroup of at most 4 waves (every hitter launch is 256
+// threads): 4 KB of LDS per workgroup, so a CU's eight hitter workgroups leave room for other
+// kernels' workgroups beside them. LDS operations of one wave complete in order, so the reads see
+// the wave's own writes without a barrier.
 __device__ __forceinline__ void scan_records16(const TriRec* __restrict__ recs, int32_t cnt, const RayLanes& L,
                                                float& best, int32_t& bi) {
-  __shared__ float4 rec_stage[16][64];
+  __shared__ float4 rec_stage[4][64];
   float4* const buf = rec_stage[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
